@@ -1,0 +1,172 @@
+/* C ABI of the gfx950 KatoANN library (libkatoann.so).
+ *
+ * This is the drop-in boundary for the batched IVF search: the entry points are
+ * what the reference's front ends bind (python/katoann_module.cpp:39-128, the
+ * CLI tools/katoann.cpp), expressed with plain pointers and sizes. The C++ API
+ * of the reference (include/katoann/index.h) is rebuilt on top of this ABI in
+ * include/katoann/index.h.
+ *
+ * Errors: every int-returning function returns KATOANN_OK or the class of the
+ * exception the reference throws (invalid_argument, runtime_error, logic_error);
+ * katoann_last_error() gives the message (thread-local).
+ * Queries, ids and dists are host memory unless stated otherwise. A loaded index
+ * owns all its device memory; concurrent searches on one index are safe.
+ */
+#ifndef KATOANN_C_H_
+#define KATOANN_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum katoann_status {
+  KATOANN_OK = 0,
+  KATOANN_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  KATOANN_RUNTIME_ERROR = 2,    /* std::runtime_error */
+  KATOANN_LOGIC_ERROR = 3       /* std::logic_error */
+};
+
+enum katoann_precision { KATOANN_SQ16 = 0, KATOANN_BF16 = 1 };          /* index.h:32-35 */
+enum katoann_ranking { KATOANN_RANK_FULL = 0, KATOANN_RANK_HIGH = 1, KATOANN_RANK_PREFIX = 2 }; /* index.h:45-49 */
+enum katoann_selection { KATOANN_SEL_THRESHOLD = 0, KATOANN_SEL_LOCAL = 1 };                   /* index.h:51-54 */
+enum katoann_engine { KATOANN_ENGINE_AMX = 0, KATOANN_ENGINE_AVX512 = 1, KATOANN_ENGINE_GFX950 = 2 }; /* kernels.h:19 */
+
+/* BuildOptions (index.h:22-30) */
+typedef struct katoann_build_options {
+  uint32_t nlist;
+  uint32_t train_size;
+  uint32_t kmeans_iters;
+  uint64_t seed;
+  int32_t normalized;
+  uint32_t rotation_sample;
+  uint32_t jacobi_sweeps;
+} katoann_build_options;
+
+/* LoadOptions (index.h:37-43); use_amx is accepted and ignored. */
+typedef struct katoann_load_options {
+  int32_t precision;
+  int32_t use_amx;
+  uint32_t prefix_dims;
+  int32_t batched_coarse;
+} katoann_load_options;
+
+/* SearchParams (index.h:56-62) */
+typedef struct katoann_search_params {
+  uint32_t k;
+  uint32_t nprobe;
+  int32_t ranking;
+  uint32_t survivors;
+  int32_t selection;
+} katoann_search_params;
+
+typedef struct katoann_index katoann_index;
+
+const char *katoann_last_error(void);
+void katoann_default_build_options(katoann_build_options *o);
+void katoann_default_load_options(katoann_load_options *o);
+void katoann_default_search_params(katoann_search_params *p);
+
+/* Index::build (index.h:74-75, src/index.cpp:419-514): host k-means and PCA,
+ * writes the index files under prefix. */
+int katoann_build(const float *data, uint32_t n, uint32_t dim, const katoann_build_options *opt,
+                  const char *prefix);
+
+int katoann_index_create(katoann_index **out);
+void katoann_index_destroy(katoann_index *h);
+
+/* Index::load (index.h:77, src/index.cpp:777-807): reads the files and builds
+ * the device-resident index (rotation, 16-bit codes, norms) on the GPU. */
+int katoann_load(katoann_index *h, const char *prefix, const katoann_load_options *opt);
+
+/* Index::search (index.h:80-81, src/index.cpp:879-904): nq x dim queries, not
+ * normalised here; ids and dists are nq x k, nearest first. */
+int katoann_search(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
+                   uint32_t *ids, float *dists);
+
+/* Accessors (index.h:83-93) */
+uint32_t katoann_dim(const katoann_index *h);
+uint32_t katoann_nlist(const katoann_index *h);
+uint32_t katoann_size(const katoann_index *h);
+int32_t katoann_normalized(const katoann_index *h);
+int32_t katoann_engine(const katoann_index *h);
+int32_t katoann_precision(const katoann_index *h);
+uint32_t katoann_prefix_dims(const katoann_index *h);
+int32_t katoann_rotated(const katoann_index *h);
+int32_t katoann_batched_coarse(const katoann_index *h);
+int katoann_list_sizes(const katoann_index *h, uint32_t *out); /* nlist entries */
+uint64_t katoann_memory_bytes(const katoann_index *h);
+
+/* ---- extensions (no reference counterpart) ----------------------------- */
+
+/* Optional inputs and outputs of one search. */
+typedef struct katoann_search_extras {
+  const uint32_t *nprobe_per_query; /* nq entries in [1, nprobe]: per-query probe count, or NULL */
+  int32_t normalize;                /* normalise the queries first, as the Python module and CLI do */
+  int32_t device_io;                /* queries, ids, dists (and the outputs below) are device pointers */
+  uint32_t *probes;                 /* nq x nprobe: probed lists, nearest first (UINT32_MAX past a query's count) */
+  float *probe_dists;               /* nq x nprobe: their coarse distances */
+  uint64_t *survivors;              /* nq x max(survivors, k): ranking keys (float bits << 32 | id), ascending */
+  uint32_t *n_survivors;            /* nq */
+  float *rotated;                   /* nq x dim: rotated (normalised) queries */
+  uint16_t *codes;                  /* nq x Dc: 16-bit query codes */
+  void *stream;                     /* hipStream_t to run on (device_io only), or NULL for the index's own */
+} katoann_search_extras;
+
+int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
+                      const katoann_search_extras *x, uint32_t *ids, float *dists);
+
+/* Load from arrays instead of files: vectors (n x dim, grouped by list in list
+ * order; host memory, or device memory when vectors_on_device), ids in the same
+ * order, list_sizes (nlist), centroids (nlist x dim) and the rotation (dim x dim,
+ * row j = basis j, or NULL) in host memory. prefix_dims as stored in .dimc. */
+int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t n, int32_t normalized,
+                        const float *centroids, const uint32_t *list_sizes, const uint32_t *ids,
+                        const float *vectors, int32_t vectors_on_device, const float *rotation,
+                        uint32_t prefix_dims, const katoann_load_options *opt);
+
+/* Scale of the codes: lo, 65535 / (hi - lo), step^2 (src/index.cpp:648-653). */
+int katoann_index_scale(const katoann_index *h, float *lo, double *inv_step, double *step_sq);
+
+/* The unsigned 16-bit codes of the loaded index (n x Dc in list order, nlist x Dc)
+ * and the ids in list order; any pointer may be NULL. */
+int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *centroid_codes, uint32_t *ids);
+
+/* Per-stage GPU times of the calling thread's last search on this index, in ms:
+ * [0] normalise + rotate + quantise, [1] coarse, [2] probe selection,
+ * [3] inversion + plan, [4] ranking, [5] survivors + refinement + top-k.
+ * Timing is recorded only while enabled. */
+#define KATOANN_NSTAGES 6
+int katoann_set_timing(katoann_index *h, int32_t on);
+int katoann_stage_times(const katoann_index *h, float *ms, int32_t n);
+/* Work of the last search: [0] (query, list) pairs, [1] ranking items,
+ * [2] candidate pairs (query x row), [3] pool entries written. */
+int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n);
+
+/* Tile parameters of the ranking pass: rows per work item (multiple of 64). */
+int katoann_set_rank_rows(katoann_index *h, uint32_t rows);
+
+/* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
+ * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of
+ * the unsigned codes; out is m x n. High-byte form: the leading padded_dim bytes
+ * of rows that are plane_dim wide (0 means padded_dim). */
+int katoann_sq16_l2(const uint8_t *qhi, const uint8_t *qlo, const uint8_t *xhi, const uint8_t *xlo,
+                    const int64_t *qnorm, const int64_t *xnorm, double step_sq, uint32_t m, uint32_t n,
+                    uint32_t padded_dim, float *out);
+int katoann_sq16_high_l2(const uint8_t *qhi, const uint8_t *xhi, const int64_t *qnorm, const int64_t *xnorm,
+                         double step_sq, uint32_t m, uint32_t n, uint32_t padded_dim, float *out,
+                         uint32_t plane_dim);
+
+/* Row normalisation of the front ends (python/katoann_module.cpp:97-110) on the GPU. */
+int katoann_normalize_rows(float *rows, uint64_t n, uint32_t dim);
+
+/* The rotation of the load and search paths (src/index.cpp:139-270) on the GPU. */
+int katoann_rotate_rows(const float *in, float *out, uint64_t n, uint32_t dim, const float *rotation);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KATOANN_C_H_ */

@@ -1,0 +1,167 @@
+// Device-side layout of a loaded index and wave-level helpers shared by the
+// gfx950 kernels. One wavefront = 64 lanes.
+//
+// Codes are stored shifted: s = u - 128 for every byte of the 16-bit code
+// u = 256 h + l (so c - 32896 = 256 (h - 128) + (l - 128)). Squared L2 is
+// unchanged by a common shift, so every integer distance equals the reference's
+// unsigned one, and gfx950's signed int8 MFMA applies without bias terms.
+// Padded dimensions hold 0 on both sides and add nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace katoann_dev {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+constexpr uint64_t kNoKey = ~0ull;   // empty slot; above every real key
+
+// f(l2) = float(double(l2) * step_sq), round to nearest even (reference
+// kernels.cpp:166-168, 209-210).
+__device__ __forceinline__ float fscale(int64_t l2, double step_sq) {
+  return static_cast<float>(static_cast<double>(l2) * step_sq);
+}
+
+// Ranking / result key: float bits (non-negative floats order like their bits)
+// above the 32-bit id, so the u64 order is the (float, id) order.
+__device__ __forceinline__ uint64_t make_key(float f, uint32_t id) {
+  return (static_cast<uint64_t>(__float_as_uint(f)) << 32) | id;
+}
+
+__device__ __forceinline__ float key_float(uint64_t k) {
+  return __uint_as_float(static_cast<uint32_t>(k >> 32));
+}
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Largest l in [0, lmax] with fscale(l) <= t (f is monotone non-decreasing);
+// -1 if none. Candidates whose integer l2 exceeds it cannot reach key t.
+__device__ inline int64_t l2_bound(float t, double step_sq, int64_t lmax) {
+  if (!(fscale(0, step_sq) <= t)) return -1;
+  if (fscale(lmax, step_sq) <= t) return lmax;
+  double g = floor(static_cast<double>(t) / step_sq);
+  int64_t lo = 0, hi = lmax;  // invariant: f(lo) <= t < f(hi)
+  if (g >= 0.0 && g < static_cast<double>(lmax)) {
+    const int64_t gi = static_cast<int64_t>(g);
+    if (fscale(gi, step_sq) <= t) {
+      lo = gi;
+      if (gi + 1 <= lmax && !(fscale(gi + 1, step_sq) <= t)) return gi;
+    } else {
+      hi = gi;
+    }
+  }
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (fscale(mid, step_sq) <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- wave-wide sorting of 64-bit keys (one key per lane, ascending) --------
+
+__device__ __forceinline__ uint64_t cmpx(uint64_t v, int j, bool keep_min) {
+  const uint64_t o = __shfl_xor(v, j);
+  return keep_min ? umin64(v, o) : umax64(v, o);
+}
+
+__device__ inline uint64_t wave_sort64(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = cmpx(v, j, lower == up);
+    }
+  }
+  return v;
+}
+
+// Sorts a bitonic sequence held one per lane into ascending order.
+__device__ inline uint64_t wave_bitonic_merge64(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) v = cmpx(v, j, (lane & j) == 0);
+  return v;
+}
+
+// Top list of up to SC keys per query, SC = 64 (one per lane) or 128 (two).
+// e[0] holds indices 0..63, e[1] indices 64..127. Empty slots are kNoKey.
+template <int SC>
+struct TopList {
+  uint64_t e[SC / 64];
+};
+
+// Keeps the S smallest of (list, c), where c holds at most one key per lane
+// (kNoKey where none). Keys are unique.
+template <int SC>
+__device__ inline void top_merge(TopList<SC> &t, uint64_t c, uint32_t S) {
+  const int lane = lane_id();
+  const uint64_t cs = wave_sort64(c);
+  const uint64_t crev = __shfl(cs, 63 - lane);
+  if (SC == 64) {
+    t.e[0] = wave_bitonic_merge64(umin64(t.e[0], crev));
+  } else {
+    uint64_t m0 = t.e[0];
+    uint64_t m1 = umin64(t.e[SC / 64 - 1], crev);
+    const uint64_t lo = umin64(m0, m1), hi = umax64(m0, m1);
+    t.e[0] = wave_bitonic_merge64(lo);
+    t.e[SC / 64 - 1] = wave_bitonic_merge64(hi);
+  }
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r)
+    if (static_cast<uint32_t>(lane + 64 * r) >= S) t.e[r] = kNoKey;
+}
+
+// Inserts one key (wave-uniform) into the sorted list, keeping S entries.
+template <int SC>
+__device__ inline void top_insert(TopList<SC> &t, uint64_t k, uint32_t S) {
+  const int lane = lane_id();
+  uint32_t pos = 0;
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) pos += __popcll(ballot(t.e[r] < k));
+  if (pos >= S) return;
+  uint64_t carry = 0;
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) {
+    const uint64_t old = t.e[r];
+    uint64_t prev = __shfl_up(old, 1);
+    if (lane == 0) prev = carry;
+    carry = __shfl(old, 63);
+    const uint32_t g = static_cast<uint32_t>(lane + 64 * r);
+    t.e[r] = g < pos ? old : (g == pos ? k : prev);
+    if (g >= S) t.e[r] = kNoKey;
+  }
+}
+
+// Entry i of the list; i may differ per lane (both registers are read from the
+// source lane, then the caller's register is picked).
+template <int SC>
+__device__ __forceinline__ uint64_t top_get(const TopList<SC> &t, uint32_t i) {
+  uint64_t v = 0;
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) {
+    const uint64_t x = __shfl(t.e[r], static_cast<int>(i % 64));
+    if (i / 64 == static_cast<uint32_t>(r)) v = x;
+  }
+  return v;
+}
+
+template <int SC>
+__device__ __forceinline__ uint32_t top_count(const TopList<SC> &t) {
+  uint32_t n = 0;
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) n += __popcll(ballot(t.e[r] != kNoKey));
+  return n;
+}
+
+}  // namespace katoann_dev

@@ -1,0 +1,848 @@
+// Host side of the gfx950 KatoANN library: index files, the device-resident
+// index, the search pipeline and the C ABI (include/katoann_c.h).
+//
+// The search runs as one stream of kernels per batch (kernels.hip):
+//   normalise? -> rotate -> quantise -> coarse MFMA -> probe select ->
+//   inversion + plan -> ranking (persistent) -> survivors + refine + top-k
+// Each concurrent search takes its own context (stream, workspace, events)
+// from a pool, so Index::search stays const and thread-safe as in the
+// reference (src/index.cpp:879-904, README.md:228).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/katoann_c.h"
+#include "kernels.hpp"
+
+namespace {
+
+constexpr uint32_t kMaxDim = 4096;          // kernels.h:24
+constexpr uint32_t kDimcMagic = 0x434D4944u;  // index.cpp:27
+constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
+constexpr uint32_t kQueriesPerItem = 64;    // ranking kernel's query chunk
+constexpr uint32_t kMaxSurvivors = 128;
+constexpr uint32_t kMaxProbes = 4096;
+
+thread_local std::string g_last_error;
+
+struct logic_err : std::logic_error {
+  using std::logic_error::logic_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) hip_check((x), #x)
+
+// Device allocation that grows on demand.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    const size_t want = std::max<size_t>(n, 256);
+    HIPCHECK(hipMalloc(&p, want));
+    bytes = want;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+template <class T>
+void upload(DevBuf &b, const T *src, size_t count) {
+  b.ensure(count * sizeof(T));
+  if (count) HIPCHECK(hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct SearchCtx {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[KATOANN_NSTAGES + 1] = {};
+  DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, gfill, item_off, qlist,
+      totals, pool_off, pool_cnt, thr, state, pool, ids, dists, surv, nsurv, codes;
+  std::vector<uint32_t> host_totals;
+  SearchCtx() {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+  }
+  ~SearchCtx() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace
+
+struct katoann_index {
+  uint32_t dim = 0, nlist = 0, npts = 0, dc = 0, head_dims = 0;
+  bool normalized = false, rotated = false, batched = true;
+  int32_t precision = KATOANN_SQ16;
+  float lo = 0.0f;
+  double inv_step = 0.0, step = 0.0, step_sq = 0.0;
+  std::vector<uint32_t> sizes;
+  std::vector<uint64_t> list_off;
+  uint64_t nrows = 0;
+  uint32_t rank_rows = 1024;
+  std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
+  uint32_t tile_prefix_rows = 0;
+
+  DevBuf rot, list_off_d, ids, id_row, hi, lo_plane, pre, norm, hinorm, headnorm, cen_hi, cen_lo, cen_norm;
+
+  std::mutex mu;
+  std::vector<std::unique_ptr<SearchCtx>> free_ctx;
+  std::atomic<bool> timing{false};
+  // last search's stage times and work, per calling thread
+  std::mutex stat_mu;
+  float stage_ms[KATOANN_NSTAGES] = {};
+  uint64_t work[4] = {};
+
+  void reset() {
+    dim = nlist = npts = dc = head_dims = 0;
+    normalized = rotated = false;
+    batched = true;
+    precision = KATOANN_SQ16;
+    lo = 0.0f;
+    inv_step = step = step_sq = 0.0;
+    sizes.clear();
+    list_off.clear();
+    nrows = 0;
+    tile_prefix.clear();
+    tile_prefix_rows = 0;
+    for (DevBuf *b : {&rot, &list_off_d, &ids, &id_row, &hi, &lo_plane, &pre, &norm, &hinorm, &headnorm, &cen_hi,
+                      &cen_lo, &cen_norm})
+      b->release();
+  }
+
+  std::unique_ptr<SearchCtx> acquire() const {
+    auto *self = const_cast<katoann_index *>(this);
+    std::lock_guard<std::mutex> g(self->mu);
+    if (!self->free_ctx.empty()) {
+      auto c = std::move(self->free_ctx.back());
+      self->free_ctx.pop_back();
+      return c;
+    }
+    return std::make_unique<SearchCtx>();
+  }
+  void release(std::unique_ptr<SearchCtx> c) const {
+    auto *self = const_cast<katoann_index *>(this);
+    std::lock_guard<std::mutex> g(self->mu);
+    self->free_ctx.push_back(std::move(c));
+  }
+
+  // Upper bound of the ranking items any one query takes part in at nprobe np.
+  uint64_t max_items_per_query(uint32_t np) {
+    std::lock_guard<std::mutex> g(mu);
+    if (tile_prefix_rows != rank_rows || tile_prefix.size() != nlist + 1u) {
+      std::vector<uint64_t> t(nlist);
+      for (uint32_t c = 0; c < nlist; ++c) t[c] = (sizes[c] + rank_rows - 1) / rank_rows;
+      std::sort(t.begin(), t.end(), std::greater<uint64_t>());
+      tile_prefix.assign(nlist + 1, 0);
+      for (uint32_t c = 0; c < nlist; ++c) tile_prefix[c + 1] = tile_prefix[c] + t[c];
+      tile_prefix_rows = rank_rows;
+    }
+    return tile_prefix[std::min(np, nlist)];
+  }
+};
+
+namespace {
+
+uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+int device_cus() {
+  static int cus = [] {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }();
+  return cus;
+}
+
+// ---------------------------------------------------------------- files
+
+struct RawIndex {
+  uint32_t dim = 0, nlist = 0, npts = 0;
+  bool normalized = false;
+  std::vector<float> centroids, vecs;
+  std::vector<uint32_t> sizes, ids;
+};
+
+// Reads .meta, .centroids, .invlist with the reference's checks and messages
+// (src/index.cpp:550-591).
+void read_files(const std::string &prefix, RawIndex &r) {
+  {
+    std::ifstream f(prefix + ".meta", std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + prefix + ".meta");
+    uint32_t meta[3] = {0, 0, 0};
+    f.read(reinterpret_cast<char *>(meta), sizeof(meta));
+    if (!f) throw std::runtime_error("truncated " + prefix + ".meta");
+    r.dim = meta[0];
+    r.nlist = meta[1];
+    r.npts = meta[2];
+    uint32_t extra[2] = {0, 0};
+    f.read(reinterpret_cast<char *>(extra), sizeof(extra));
+    if (f.gcount() == static_cast<std::streamsize>(sizeof(extra))) {
+      if (extra[0] != 0) throw std::runtime_error("only the L2 metric is supported");
+      r.normalized = extra[1] != 0;
+    }
+  }
+  if (r.dim == 0 || r.dim > kMaxDim || r.nlist == 0)
+    throw std::runtime_error("bad header in " + prefix + ".meta");
+  r.centroids.assign(static_cast<size_t>(r.nlist) * r.dim, 0.0f);
+  {
+    std::ifstream f(prefix + ".centroids", std::ios::binary);
+    f.read(reinterpret_cast<char *>(r.centroids.data()),
+           static_cast<std::streamsize>(r.centroids.size() * sizeof(float)));
+    if (!f) throw std::runtime_error("cannot read " + prefix + ".centroids");
+  }
+  std::ifstream f(prefix + ".invlist", std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + prefix + ".invlist");
+  r.sizes.assign(r.nlist, 0);
+  uint64_t total = 0;
+  r.ids.clear();
+  r.vecs.clear();
+  for (uint32_t c = 0; c < r.nlist; ++c) {
+    uint32_t sz = 0;
+    f.read(reinterpret_cast<char *>(&sz), sizeof(sz));
+    if (!f) break;
+    r.sizes[c] = sz;
+    const size_t i0 = r.ids.size(), v0 = r.vecs.size();
+    r.ids.resize(i0 + sz);
+    f.read(reinterpret_cast<char *>(r.ids.data() + i0), static_cast<std::streamsize>(sz) * 4);
+    r.vecs.resize(v0 + static_cast<size_t>(sz) * r.dim);
+    f.read(reinterpret_cast<char *>(r.vecs.data() + v0), static_cast<std::streamsize>(sz) * r.dim * 4);
+    total += sz;
+    if (!f) break;
+  }
+  if (!f || total != r.npts) throw std::runtime_error("truncated " + prefix + ".invlist");
+}
+
+// .dimc header and rotation (src/index.cpp:593-611); false when absent.
+bool read_rotation(const std::string &prefix, const RawIndex &r, uint32_t *stored, bool *rotated,
+                   std::vector<float> &rotation) {
+  std::ifstream f(prefix + ".dimc", std::ios::binary);
+  if (!f) return false;
+  uint32_t hdr[8] = {0};
+  f.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+  if (!f || hdr[0] != kDimcMagic || (hdr[1] != 1u && hdr[1] != kDimcVersion))
+    throw std::runtime_error("bad header in " + prefix + ".dimc");
+  if (hdr[2] != r.dim || hdr[3] != r.nlist || hdr[4] != r.npts)
+    throw std::runtime_error(prefix + ".dimc does not match the index");
+  *stored = hdr[5];
+  *rotated = hdr[6] == 1u;
+  if (*rotated) {
+    rotation.resize(static_cast<size_t>(r.dim) * r.dim);
+    f.read(reinterpret_cast<char *>(rotation.data()), static_cast<std::streamsize>(rotation.size() * 4));
+    if (!f) throw std::runtime_error("truncated rotation in " + prefix + ".dimc");
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- load
+
+void check_load_options(const katoann_load_options &o) {
+  if (o.precision == KATOANN_BF16)
+    throw std::invalid_argument("bf16 precision is not supported by the gfx950 engine");
+  if (o.precision != KATOANN_SQ16) throw std::invalid_argument("precision is 'sq16' or 'bf16'");
+  if (!o.batched_coarse)
+    throw std::invalid_argument("per-query flat coarse search is not supported by the gfx950 engine");
+}
+
+// Everything of Index::load after reading the files (src/index.cpp:777-807):
+// rotation of every list, the global scale, the 16-bit codes and norms, the
+// prefix plane, the centroid codes and the id map, all on the GPU.
+void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_t npts, bool normalized,
+                        const float *centroids, const uint32_t *sizes, const uint32_t *ids, const float *vecs,
+                        bool vecs_on_device, const float *rotation, bool has_dimc, uint32_t dims) {
+  ix.reset();
+  ix.dim = dim;
+  ix.nlist = nlist;
+  ix.npts = npts;
+  ix.normalized = normalized;
+  ix.dc = round_up(dim, 64u);
+  ix.sizes.assign(sizes, sizes + nlist);
+  ix.list_off.assign(nlist + 1, 0);
+  for (uint32_t c = 0; c < nlist; ++c) ix.list_off[c + 1] = ix.list_off[c] + sizes[c];
+  const uint64_t N = ix.list_off[nlist];
+  ix.nrows = N;
+  if (has_dimc && dims) {
+    if (dims % 64 || dims > ix.dc)
+      throw std::invalid_argument("prefix_dims must be a multiple of 64 within the dimension");
+    ix.head_dims = dims;
+  }
+  ix.rotated = rotation != nullptr;
+  hipStream_t s = nullptr;
+  const uint32_t d = dim, dc = ix.dc;
+
+  DevBuf raw, y;
+  const float *dv = vecs;
+  if (!vecs_on_device && N) {
+    upload(raw, vecs, static_cast<size_t>(N) * d);
+    dv = raw.as<float>();
+  }
+  const float *yv = dv;
+  if (ix.rotated) {
+    upload(ix.rot, rotation, static_cast<size_t>(d) * d);
+    if (N) {
+      y.ensure(static_cast<size_t>(N) * d * 4);
+      // rows in chunks so each launch stays within 32-bit grid sizes
+      const uint64_t chunk = 1u << 22;
+      for (uint64_t r0 = 0; r0 < N; r0 += chunk) {
+        const uint32_t nr = static_cast<uint32_t>(std::min<uint64_t>(chunk, N - r0));
+        katoann_hip::launch_rotate(dv + r0 * d, nr, d, ix.rot.as<float>(), y.as<float>() + r0 * d, s);
+      }
+      HIPCHECK(hipGetLastError());
+      yv = y.as<float>();
+    }
+    raw.release();
+  }
+  // global scale (src/index.cpp:634-653)
+  float glo = std::numeric_limits<float>::max(), ghi = -std::numeric_limits<float>::max();
+  if (N) {
+    const uint32_t nb = 2048;
+    DevBuf part;
+    part.ensure(nb * 2 * sizeof(float));
+    katoann_hip::launch_minmax(yv, N * d, part.as<float>(), nb, s);
+    std::vector<float> hp(nb * 2);
+    HIPCHECK(hipMemcpy(hp.data(), part.p, hp.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < nb; ++b) {
+      glo = std::min(glo, hp[2 * b]);
+      ghi = std::max(ghi, hp[2 * b + 1]);
+    }
+  }
+  ix.lo = glo;
+  const double rng = static_cast<double>(ghi) - glo;
+  ix.inv_step = 65535.0 / rng;
+  ix.step = rng / 65535.0;
+  ix.step_sq = ix.step * ix.step;
+
+  const uint32_t P = ix.head_dims;
+  ix.hi.ensure(static_cast<size_t>(N) * dc);
+  ix.lo_plane.ensure(static_cast<size_t>(N) * dc);
+  ix.norm.ensure(N * 8);
+  ix.hinorm.ensure(N * 4);
+  ix.headnorm.ensure(N * 4);
+  if (P) ix.pre.ensure(static_cast<size_t>(N) * P);
+  if (N) {
+    const uint64_t chunk = 1u << 22;
+    for (uint64_t r0 = 0; r0 < N; r0 += chunk) {
+      const uint32_t nr = static_cast<uint32_t>(std::min<uint64_t>(chunk, N - r0));
+      katoann_hip::launch_quantize(yv + r0 * d, nr, d, dc, P, ix.lo, ix.inv_step, ix.hi.as<int8_t>() + r0 * dc,
+                                   ix.lo_plane.as<int8_t>() + r0 * dc, ix.norm.as<int64_t>() + r0,
+                                   ix.hinorm.as<int32_t>() + r0, ix.headnorm.as<int32_t>() + r0,
+                                   P ? ix.pre.as<int8_t>() + r0 * P : nullptr, nullptr, s);
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  y.release();
+  raw.release();
+
+  // centroids with the database scale (src/index.cpp:718-750)
+  {
+    DevBuf c, cy, tmp32;
+    upload(c, centroids, static_cast<size_t>(nlist) * d);
+    const float *cv = c.as<float>();
+    if (ix.rotated) {
+      cy.ensure(static_cast<size_t>(nlist) * d * 4);
+      katoann_hip::launch_rotate(cv, nlist, d, ix.rot.as<float>(), cy.as<float>(), s);
+      cv = cy.as<float>();
+    }
+    ix.cen_hi.ensure(static_cast<size_t>(nlist) * dc);
+    ix.cen_lo.ensure(static_cast<size_t>(nlist) * dc);
+    ix.cen_norm.ensure(static_cast<size_t>(nlist) * 8);
+    katoann_hip::launch_quantize(cv, nlist, d, dc, 0, ix.lo, ix.inv_step, ix.cen_hi.as<int8_t>(),
+                                 ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nullptr, nullptr, nullptr,
+                                 nullptr, s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
+  }
+  // ids and id -> row (src/index.cpp:795-806; a later duplicate wins)
+  upload(ix.ids, ids, N);
+  {
+    std::vector<uint32_t> idr(npts, UINT32_MAX);
+    for (uint64_t r = 0; r < N; ++r)
+      if (ids[r] < npts) idr[ids[r]] = static_cast<uint32_t>(r);
+    upload(ix.id_row, idr.data(), idr.size());
+  }
+  upload(ix.list_off_d, ix.list_off.data(), ix.list_off.size());
+  HIPCHECK(hipDeviceSynchronize());
+}
+
+// ---------------------------------------------------------------- search
+
+void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
+                 const katoann_search_extras *x, uint32_t *out_ids, float *out_dists) {
+  if (nq == 0) return;
+  if (ix.nlist == 0) throw logic_err("search before load");
+  if (p.k == 0 || p.nprobe == 0 || p.nprobe > ix.nlist)
+    throw std::invalid_argument("need k >= 1 and 1 <= nprobe <= nlist");
+  if (p.ranking == KATOANN_RANK_PREFIX && ix.head_dims == 0)
+    throw std::invalid_argument("prefix ranking needs the rotation file (.dimc)");
+  if (p.ranking == KATOANN_RANK_FULL)
+    throw std::invalid_argument("Ranking::kFull is not supported by the gfx950 engine yet");
+  if (p.ranking != KATOANN_RANK_PREFIX && p.ranking != KATOANN_RANK_HIGH)
+    throw std::invalid_argument("ranking is 'prefix', 'high' or 'full'");
+  if (p.selection != KATOANN_SEL_THRESHOLD && p.selection != KATOANN_SEL_LOCAL)
+    throw std::invalid_argument("selection is 'threshold' or 'local'");
+  const uint32_t S = std::max(p.survivors, p.k);
+  if (S > kMaxSurvivors)
+    throw std::invalid_argument("max(survivors, k) above 128 is not supported by the gfx950 engine yet");
+  if (p.nprobe > kMaxProbes) throw std::invalid_argument("nprobe above 4096 is not supported by the gfx950 engine");
+  const bool prefix = p.ranking == KATOANN_RANK_PREFIX;
+  const uint32_t P = prefix ? ix.head_dims : ix.dc;
+  if (P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
+
+  const uint32_t B = nq, d = ix.dim, dc = ix.dc, np = p.nprobe, K = p.k, nlist = ix.nlist;
+  const bool dev_io = x && x->device_io;
+  auto &mix = const_cast<katoann_index &>(ix);
+  auto ctx = ix.acquire();
+  SearchCtx &c = *ctx;
+  hipStream_t s = (dev_io && x->stream) ? static_cast<hipStream_t>(x->stream) : c.stream;
+  const bool timing = ix.timing.load();
+
+  c.q.ensure(static_cast<size_t>(B) * d * 4);
+  c.y.ensure(static_cast<size_t>(B) * d * 4);
+  c.qhi.ensure(static_cast<size_t>(B) * dc);
+  c.qlo.ensure(static_cast<size_t>(B) * dc);
+  c.qnorm.ensure(B * 8ull);
+  c.qhn.ensure(B * 4ull);
+  c.qh0.ensure(B * 4ull);
+  c.dist.ensure(static_cast<size_t>(B) * nlist * 4);
+  c.probes.ensure(static_cast<size_t>(B) * np * 4);
+  c.probe_d.ensure(static_cast<size_t>(B) * np * 4);
+  c.gcnt.ensure(nlist * 4ull);
+  c.qoff.ensure(nlist * 4ull);
+  c.gfill.ensure(nlist * 4ull);
+  c.item_off.ensure((nlist + 1ull) * 4);
+  c.qlist.ensure(static_cast<size_t>(B) * np * 4);
+  c.totals.ensure(16);
+  c.pool_off.ensure((B + 1ull) * 8);
+  c.pool_cnt.ensure(B * 4ull);
+  c.thr.ensure(B * 8ull);
+  c.state.ensure(16);
+  const uint32_t rt = ix.rank_rows;
+  const uint64_t cap = static_cast<uint64_t>(S) * B * mix.max_items_per_query(np);
+  c.pool.ensure(std::max<uint64_t>(cap, 1) * 8);
+  c.ids.ensure(static_cast<size_t>(B) * K * 4);
+  c.dists.ensure(static_cast<size_t>(B) * K * 4);
+  if (x && x->survivors) c.surv.ensure(static_cast<size_t>(B) * S * 8);
+  if (x && x->n_survivors) c.nsurv.ensure(B * 4ull);
+  if (x && x->codes) c.codes.ensure(static_cast<size_t>(B) * dc * 2);
+  const uint32_t *npq = nullptr;
+  if (x && x->nprobe_per_query) {
+    c.npq.ensure(B * 4ull);
+    HIPCHECK(hipMemcpyAsync(c.npq.p, x->nprobe_per_query, B * 4ull,
+                            dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    npq = c.npq.as<uint32_t>();
+  }
+  auto mark = [&](int i) {
+    if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
+  };
+
+  HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
+                          dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  mark(0);
+  if (x && x->normalize) katoann_hip::launch_normalize(c.q.as<float>(), B, d, s);
+  const float *yq = c.q.as<float>();
+  if (ix.rotated) {
+    katoann_hip::launch_rotate(c.q.as<float>(), B, d, ix.rot.as<float>(), c.y.as<float>(), s);
+    yq = c.y.as<float>();
+  }
+  katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
+                               c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(), c.qh0.as<int32_t>(),
+                               nullptr, (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
+  mark(1);
+  katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
+                             ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist, dc,
+                             ix.step_sq, c.dist.as<float>(), s);
+  mark(2);
+  katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(), s);
+  mark(3);
+  katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
+                           kQueriesPerItem, rt, S, c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(),
+                           c.item_off.as<uint32_t>(), c.gfill.as<uint32_t>(), c.qlist.as<uint32_t>(),
+                           c.totals.as<uint32_t>(), c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(),
+                           c.thr.as<uint64_t>(), c.state.as<uint32_t>(), s);
+  mark(4);
+  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 2;
+  if (!katoann_hip::launch_rank(prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>(),
+                                prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>(), ix.ids.as<uint32_t>(),
+                                ix.list_off_d.as<uint64_t>(), nlist, P, c.qhi.as<int8_t>(), dc,
+                                prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>(), c.gcnt.as<uint32_t>(),
+                                c.qoff.as<uint32_t>(), c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
+                                c.totals.as<uint32_t>(), rt, S, ix.step_sq, c.pool.as<uint64_t>(),
+                                c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
+                                c.state.as<uint32_t>(), grid, s))
+    throw std::invalid_argument("unsupported ranking shape");
+  mark(5);
+  uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
+  float *ddst = dev_io ? out_dists : c.dists.as<float>();
+  katoann_hip::launch_refine(c.pool.as<uint64_t>(), c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(),
+                             c.thr.as<uint64_t>(), B, S, K, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(),
+                             ix.norm.as<int64_t>(), ix.id_row.as<uint32_t>(), ix.npts, dc, c.qhi.as<int8_t>(),
+                             c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), ix.step_sq, dids, ddst,
+                             (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
+                             (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s);
+  mark(6);
+  HIPCHECK(hipGetLastError());
+  const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (!dev_io) {
+    HIPCHECK(hipMemcpyAsync(out_ids, dids, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(out_dists, ddst, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
+  }
+  if (x) {
+    if (x->probes) HIPCHECK(hipMemcpyAsync(x->probes, c.probes.p, static_cast<size_t>(B) * np * 4, d2o, s));
+    if (x->probe_dists)
+      HIPCHECK(hipMemcpyAsync(x->probe_dists, c.probe_d.p, static_cast<size_t>(B) * np * 4, d2o, s));
+    if (x->survivors) HIPCHECK(hipMemcpyAsync(x->survivors, c.surv.p, static_cast<size_t>(B) * S * 8, d2o, s));
+    if (x->n_survivors) HIPCHECK(hipMemcpyAsync(x->n_survivors, c.nsurv.p, B * 4ull, d2o, s));
+    if (x->rotated) HIPCHECK(hipMemcpyAsync(x->rotated, yq, static_cast<size_t>(B) * d * 4, d2o, s));
+    if (x->codes) HIPCHECK(hipMemcpyAsync(x->codes, c.codes.p, static_cast<size_t>(B) * dc * 2, d2o, s));
+  }
+  uint32_t st[4] = {0, 0, 0, 0};
+  HIPCHECK(hipMemcpyAsync(st, c.state.p, 8, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(st + 2, c.totals.p, 8, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (x && npq && !dev_io) {  // past a query's own count: no list
+    for (uint32_t q = 0; q < B; ++q)
+      for (uint32_t j = x->nprobe_per_query[q]; j < np; ++j) {
+        if (x->probes) x->probes[static_cast<size_t>(q) * np + j] = UINT32_MAX;
+        if (x->probe_dists) x->probe_dists[static_cast<size_t>(q) * np + j] = std::numeric_limits<float>::max();
+      }
+  }
+  {
+    std::lock_guard<std::mutex> g(mix.stat_mu);
+    if (timing)
+      for (int i = 0; i < KATOANN_NSTAGES; ++i) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c.ev[i], c.ev[i + 1]);
+        mix.stage_ms[i] = ms;
+      }
+    mix.work[0] = st[2];
+    mix.work[1] = st[3];
+  }
+  const bool overflow = st[1] != 0;
+  ix.release(std::move(ctx));
+  if (overflow) throw std::runtime_error("survivor pool overflow in the ranking pass");
+}
+
+// Row sums of shifted bytes for the kernel-level API's exact unsigned products.
+void kernel_pairs(const uint8_t *qh, const uint8_t *ql, const uint8_t *xh, const uint8_t *xl, const int64_t *qn,
+                  const int64_t *xn, double step_sq, uint32_t m, uint32_t n, uint32_t dc, uint32_t qstride,
+                  uint32_t xstride, bool full, float *out);
+
+}  // namespace
+
+// ============================================================================ C ABI
+
+// The calling thread's error message, shared with build.cpp.
+std::string &katoann_error_slot() { return g_last_error; }
+
+#define KATOANN_TRY try {
+#define KATOANN_CATCH                                  \
+  return KATOANN_OK;                                   \
+  }                                                    \
+  catch (const std::invalid_argument &e) {             \
+    g_last_error = e.what();                           \
+    return KATOANN_INVALID_ARGUMENT;                   \
+  }                                                    \
+  catch (const logic_err &e) {                         \
+    g_last_error = e.what();                           \
+    return KATOANN_LOGIC_ERROR;                        \
+  }                                                    \
+  catch (const std::logic_error &e) {                  \
+    g_last_error = e.what();                           \
+    return KATOANN_LOGIC_ERROR;                        \
+  }                                                    \
+  catch (const std::exception &e) {                    \
+    g_last_error = e.what();                           \
+    return KATOANN_RUNTIME_ERROR;                      \
+  }
+
+extern "C" {
+
+const char *katoann_last_error(void) { return g_last_error.c_str(); }
+
+void katoann_default_build_options(katoann_build_options *o) {
+  *o = katoann_build_options{2048u, 256000u, 12u, 0ull, 0, 100000u, 8u};
+}
+void katoann_default_load_options(katoann_load_options *o) { *o = katoann_load_options{KATOANN_SQ16, 1, 0u, 1}; }
+void katoann_default_search_params(katoann_search_params *p) {
+  *p = katoann_search_params{10u, 64u, KATOANN_RANK_PREFIX, 32u, KATOANN_SEL_THRESHOLD};
+}
+
+int katoann_index_create(katoann_index **out) {
+  KATOANN_TRY
+  *out = new katoann_index();
+  KATOANN_CATCH
+}
+
+void katoann_index_destroy(katoann_index *h) { delete h; }
+
+int katoann_load(katoann_index *h, const char *prefix_c, const katoann_load_options *opt) {
+  KATOANN_TRY
+  katoann_load_options o;
+  katoann_default_load_options(&o);
+  if (opt) o = *opt;
+  h->reset();  // Index::load starts from an empty index (src/index.cpp:778)
+  check_load_options(o);
+  const std::string prefix(prefix_c);
+  RawIndex r;
+  read_files(prefix, r);
+  uint32_t stored = 0;
+  bool rotated = false;
+  std::vector<float> rotation;
+  const bool has_dimc = read_rotation(prefix, r, &stored, &rotated, rotation);
+  const uint32_t dims = o.prefix_dims ? o.prefix_dims : stored;
+  build_device_index(*h, r.dim, r.nlist, r.npts, r.normalized, r.centroids.data(), r.sizes.data(), r.ids.data(),
+                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims);
+  KATOANN_CATCH
+}
+
+int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t n, int32_t normalized,
+                        const float *centroids, const uint32_t *list_sizes, const uint32_t *ids,
+                        const float *vectors, int32_t vectors_on_device, const float *rotation,
+                        uint32_t prefix_dims, const katoann_load_options *opt) {
+  KATOANN_TRY
+  katoann_load_options o;
+  katoann_default_load_options(&o);
+  if (opt) o = *opt;
+  h->reset();
+  check_load_options(o);
+  if (dim == 0 || dim > kMaxDim || nlist == 0) throw std::runtime_error("bad header in arrays");
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < nlist; ++c) total += list_sizes[c];
+  if (total != n) throw std::runtime_error("list sizes do not add up to n");
+  const uint32_t dims = o.prefix_dims ? o.prefix_dims : prefix_dims;
+  build_device_index(*h, dim, nlist, n, normalized != 0, centroids, list_sizes, ids, vectors,
+                     vectors_on_device != 0, rotation, true, dims);
+  KATOANN_CATCH
+}
+
+int katoann_search(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
+                   uint32_t *ids, float *dists) {
+  KATOANN_TRY
+  search_impl(*h, queries, nq, *p, nullptr, ids, dists);
+  KATOANN_CATCH
+}
+
+int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
+                      const katoann_search_extras *x, uint32_t *ids, float *dists) {
+  KATOANN_TRY
+  if (x && x->nprobe_per_query && !x->device_io)
+    for (uint32_t q = 0; q < nq; ++q)
+      if (x->nprobe_per_query[q] == 0 || x->nprobe_per_query[q] > p->nprobe)
+        throw std::invalid_argument("per-query nprobe must be in [1, nprobe]");
+  search_impl(*h, queries, nq, *p, x, ids, dists);
+  KATOANN_CATCH
+}
+
+uint32_t katoann_dim(const katoann_index *h) { return h->dim; }
+uint32_t katoann_nlist(const katoann_index *h) { return h->nlist; }
+uint32_t katoann_size(const katoann_index *h) { return h->npts; }
+int32_t katoann_normalized(const katoann_index *h) { return h->normalized ? 1 : 0; }
+int32_t katoann_engine(const katoann_index *) { return KATOANN_ENGINE_GFX950; }
+int32_t katoann_precision(const katoann_index *h) { return h->precision; }
+uint32_t katoann_prefix_dims(const katoann_index *h) { return h->head_dims; }
+int32_t katoann_rotated(const katoann_index *h) { return h->rotated ? 1 : 0; }
+int32_t katoann_batched_coarse(const katoann_index *h) { return h->nlist ? 1 : 0; }
+
+int katoann_list_sizes(const katoann_index *h, uint32_t *out) {
+  KATOANN_TRY
+  std::copy(h->sizes.begin(), h->sizes.end(), out);
+  KATOANN_CATCH
+}
+
+uint64_t katoann_memory_bytes(const katoann_index *h) {
+  uint64_t s = 0;
+  for (const DevBuf *b : {&h->rot, &h->list_off_d, &h->ids, &h->id_row, &h->hi, &h->lo_plane, &h->pre, &h->norm,
+                          &h->hinorm, &h->headnorm, &h->cen_hi, &h->cen_lo, &h->cen_norm})
+    s += b->bytes;
+  return s;
+}
+
+int katoann_index_scale(const katoann_index *h, float *lo, double *inv_step, double *step_sq) {
+  KATOANN_TRY
+  if (h->nlist == 0) throw logic_err("search before load");
+  *lo = h->lo;
+  *inv_step = h->inv_step;
+  *step_sq = h->step_sq;
+  KATOANN_CATCH
+}
+
+int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *centroid_codes, uint32_t *ids) {
+  KATOANN_TRY
+  if (h->nlist == 0) throw logic_err("search before load");
+  const uint64_t N = h->nrows;
+  const uint32_t dc = h->dc;
+  auto unshift = [&](const DevBuf &hb, const DevBuf &lb, uint64_t rows, uint16_t *out) {
+    std::vector<int8_t> a(static_cast<size_t>(rows) * dc), b(a.size());
+    if (rows) {
+      HIPCHECK(hipMemcpy(a.data(), hb.p, a.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(b.data(), lb.p, b.size(), hipMemcpyDeviceToHost));
+    }
+    const uint32_t d = h->dim;
+    for (uint64_t r = 0; r < rows; ++r)
+      for (uint32_t k = 0; k < dc; ++k) {
+        const size_t i = static_cast<size_t>(r) * dc + k;
+        out[i] = k < d ? static_cast<uint16_t>(((a[i] + 128) << 8) | (b[i] + 128)) : 0;
+      }
+  };
+  if (codes) unshift(h->hi, h->lo_plane, N, codes);
+  if (centroid_codes) unshift(h->cen_hi, h->cen_lo, h->nlist, centroid_codes);
+  if (ids && N) HIPCHECK(hipMemcpy(ids, h->ids.p, N * 4, hipMemcpyDeviceToHost));
+  KATOANN_CATCH
+}
+
+int katoann_set_timing(katoann_index *h, int32_t on) {
+  h->timing.store(on != 0);
+  return KATOANN_OK;
+}
+
+int katoann_stage_times(const katoann_index *h, float *ms, int32_t n) {
+  auto *m = const_cast<katoann_index *>(h);
+  std::lock_guard<std::mutex> g(m->stat_mu);
+  for (int i = 0; i < n && i < KATOANN_NSTAGES; ++i) ms[i] = h->stage_ms[i];
+  return KATOANN_OK;
+}
+
+int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n) {
+  auto *m = const_cast<katoann_index *>(h);
+  std::lock_guard<std::mutex> g(m->stat_mu);
+  for (int i = 0; i < n && i < 4; ++i) out[i] = h->work[i];
+  return KATOANN_OK;
+}
+
+int katoann_set_rank_rows(katoann_index *h, uint32_t rows) {
+  KATOANN_TRY
+  if (rows == 0 || rows % 64) throw std::invalid_argument("rank rows must be a positive multiple of 64");
+  std::lock_guard<std::mutex> g(h->mu);
+  h->rank_rows = rows;
+  KATOANN_CATCH
+}
+
+int katoann_normalize_rows(float *rows, uint64_t n, uint32_t dim) {
+  KATOANN_TRY
+  DevBuf b;
+  upload(b, rows, n * dim);
+  katoann_hip::launch_normalize(b.as<float>(), static_cast<uint32_t>(n), dim, nullptr);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(rows, b.p, n * dim * 4, hipMemcpyDeviceToHost));
+  KATOANN_CATCH
+}
+
+int katoann_rotate_rows(const float *in, float *out, uint64_t n, uint32_t dim, const float *rotation) {
+  KATOANN_TRY
+  DevBuf a, r, y;
+  upload(a, in, n * dim);
+  upload(r, rotation, static_cast<size_t>(dim) * dim);
+  y.ensure(n * dim * 4);
+  katoann_hip::launch_rotate(a.as<float>(), static_cast<uint32_t>(n), dim, r.as<float>(), y.as<float>(), nullptr);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out, y.p, n * dim * 4, hipMemcpyDeviceToHost));
+  KATOANN_CATCH
+}
+
+int katoann_sq16_l2(const uint8_t *qhi, const uint8_t *qlo, const uint8_t *xhi, const uint8_t *xlo,
+                    const int64_t *qnorm, const int64_t *xnorm, double step_sq, uint32_t m, uint32_t n,
+                    uint32_t padded_dim, float *out) {
+  KATOANN_TRY
+  if (m == 0 || n == 0) return KATOANN_OK;
+  if (padded_dim == 0 || padded_dim % 64) throw std::invalid_argument("padded_dim must be a multiple of 64");
+  kernel_pairs(qhi, qlo, xhi, xlo, qnorm, xnorm, step_sq, m, n, padded_dim, padded_dim, padded_dim, true, out);
+  KATOANN_CATCH
+}
+
+int katoann_sq16_high_l2(const uint8_t *qhi, const uint8_t *xhi, const int64_t *qnorm, const int64_t *xnorm,
+                         double step_sq, uint32_t m, uint32_t n, uint32_t padded_dim, float *out,
+                         uint32_t plane_dim) {
+  KATOANN_TRY
+  if (m == 0 || n == 0) return KATOANN_OK;
+  if (plane_dim == 0) plane_dim = padded_dim;
+  if (plane_dim < padded_dim || plane_dim % 64)
+    throw std::invalid_argument("plane_dim must be a multiple of 64 and at least padded_dim");
+  if (padded_dim == 0 || padded_dim % 64) throw std::invalid_argument("padded_dim must be a multiple of 64");
+  kernel_pairs(qhi, nullptr, xhi, nullptr, qnorm, xnorm, step_sq, m, n, padded_dim, padded_dim, plane_dim, false,
+               out);
+  KATOANN_CATCH
+}
+
+}  // extern "C"
+
+namespace {
+
+// dot of unsigned codes from the shifted-byte MFMA product:
+//   full: sum cq cx = dot' + 32896 (sum c'q + sum c'x) + dc * 32896^2
+//   high: sum hq hx = dot' + 128 (sum s_q + sum s_x) + dc * 128^2
+void kernel_pairs(const uint8_t *qh, const uint8_t *ql, const uint8_t *xh, const uint8_t *xl, const int64_t *qn,
+                  const int64_t *xn, double step_sq, uint32_t m, uint32_t n, uint32_t dc, uint32_t qstride,
+                  uint32_t xstride, bool full, float *out) {
+  auto shift = [&](const uint8_t *src, uint32_t rows, uint32_t stride) {
+    std::vector<int8_t> v(static_cast<size_t>(rows) * stride);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = static_cast<int8_t>(static_cast<int>(src[i]) - 128);
+    return v;
+  };
+  auto sums = [&](const uint8_t *h, const uint8_t *l, uint32_t rows, uint32_t stride) {
+    std::vector<int64_t> s(rows, 0);
+    for (uint32_t r = 0; r < rows; ++r)
+      for (uint32_t k = 0; k < dc; ++k) {
+        const int64_t hv = h[static_cast<size_t>(r) * stride + k];
+        s[r] += full ? (hv * 256 + l[static_cast<size_t>(r) * stride + k]) - 32896 : hv - 128;
+      }
+    return s;
+  };
+  const auto sqh = shift(qh, m, qstride), sxh = shift(xh, n, xstride);
+  std::vector<int8_t> sql, sxl;
+  if (full) {
+    sql = shift(ql, m, qstride);
+    sxl = shift(xl, n, xstride);
+  }
+  const auto qs = sums(qh, ql, m, qstride), xs = sums(xh, xl, n, xstride);
+  const int64_t K = full ? 32896 : 128;
+  // fold the bias into the norms: l2 = qn + xn - 2 (dot' + K (sq + sx) + dc K^2)
+  std::vector<int64_t> qn2(m), xn2(n);
+  for (uint32_t i = 0; i < m; ++i) qn2[i] = qn[i] - 2 * K * qs[i] - static_cast<int64_t>(dc) * K * K;
+  for (uint32_t j = 0; j < n; ++j) xn2[j] = xn[j] - 2 * K * xs[j] - static_cast<int64_t>(dc) * K * K;
+  DevBuf a, b, c, e, f, g, o;
+  upload(a, sqh.data(), sqh.size());
+  upload(b, sxh.data(), sxh.size());
+  if (full) {
+    upload(c, sql.data(), sql.size());
+    upload(e, sxl.data(), sxl.size());
+  }
+  upload(f, qn2.data(), m);
+  upload(g, xn2.data(), n);
+  o.ensure(static_cast<size_t>(m) * n * 4);
+  katoann_hip::launch_sq16_pairs(a.as<int8_t>(), full ? c.as<int8_t>() : a.as<int8_t>(), f.as<int64_t>(),
+                                 b.as<int8_t>(), full ? e.as<int8_t>() : b.as<int8_t>(), g.as<int64_t>(), m, n, dc,
+                                 qstride, xstride, full ? 1 : 0, step_sq, o.as<float>(), nullptr);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out, o.p, static_cast<size_t>(m) * n * 4, hipMemcpyDeviceToHost));
+}
+
+}  // namespace

@@ -1,0 +1,42 @@
+// Host launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace katoann_hip {
+
+void launch_normalize(float *x, uint32_t n, uint32_t d, hipStream_t s);
+void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float *Y, hipStream_t s);
+void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32_t P, float lo,
+                     double inv_step, int8_t *hi, int8_t *lop, int64_t *norm, int32_t *hinorm,
+                     int32_t *headnorm, int8_t *pre, uint16_t *codes, hipStream_t s);
+void launch_minmax(const float *y, uint64_t n, float *part, uint32_t nblocks, hipStream_t s);
+void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, uint32_t B, const int8_t *chi,
+                   const int8_t *clo, const int64_t *cnorm, uint32_t nlist, uint32_t dc, double step_sq,
+                   float *dist, hipStream_t s);
+size_t select_smem(uint32_t nlist, uint32_t np);
+void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
+                   hipStream_t s);
+void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
+                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t S, uint32_t *gcnt, uint32_t *qoff,
+                 uint32_t *item_off, uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint64_t *pool_off,
+                 uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, hipStream_t s);
+size_t rank_smem(uint32_t P, uint32_t S);
+bool launch_rank(const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, const uint64_t *list_off,
+                 uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                 const uint32_t *gcnt, const uint32_t *qoff, const uint32_t *qlist, const uint32_t *item_off,
+                 const uint32_t *totals, uint32_t rt, uint32_t S, double step_sq, uint64_t *pool,
+                 const uint64_t *pool_off, uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, uint32_t grid,
+                 hipStream_t s);
+bool launch_refine(const uint64_t *pool, const uint64_t *pool_off, const uint32_t *pool_cnt, const uint64_t *thr,
+                   uint32_t B, uint32_t S, uint32_t K, const int8_t *hi, const int8_t *lo, const int64_t *norm,
+                   const uint32_t *id_row, uint32_t npts, uint32_t dc, const int8_t *qhi, const int8_t *qlo,
+                   const int64_t *qnorm, double step_sq, uint32_t *out_ids, float *out_dists, uint64_t *surv,
+                   uint32_t *nsurv, hipStream_t s);
+void launch_sq16_pairs(const int8_t *qh, const int8_t *ql, const int64_t *qn, const int8_t *xh, const int8_t *xl,
+                       const int64_t *xn, uint32_t m, uint32_t n, uint32_t dc, uint32_t qstride, uint32_t xstride,
+                       int full, double step_sq, float *out, hipStream_t s);
+
+}  // namespace katoann_hip

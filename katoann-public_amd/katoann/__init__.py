@@ -1,0 +1,344 @@
+"""KatoANN on AMD Instinct MI355X (gfx950): the Python module of the reference
+(python/katoann_module.cpp) over the library's C ABI (include/katoann_c.h).
+
+    import katoann
+    katoann.build(data, "idx/base", nlist=2048)
+    ix = katoann.Index(); ix.load("idx/base")
+    ids, dists = ix.search(queries, k=10, nprobe=64)
+
+Same names, defaults, argument meaning and error classes as the reference
+module: ValueError for std::invalid_argument, RuntimeError otherwise. The
+search runs on the GPU; there is no CPU fallback, and importing fails loudly
+when libkatoann.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import builder, formats  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libkatoann.so")
+
+RANKING = {"full": 0, "high": 1, "prefix": 2}
+SELECTION = {"threshold": 0, "local": 1}
+NSTAGES = 6
+STAGES = ("rotate+quantize", "coarse", "select", "plan", "rank", "refine")
+
+
+class _BuildOptions(C.Structure):
+    _fields_ = [("nlist", C.c_uint32), ("train_size", C.c_uint32), ("kmeans_iters", C.c_uint32),
+                ("seed", C.c_uint64), ("normalized", C.c_int32), ("rotation_sample", C.c_uint32),
+                ("jacobi_sweeps", C.c_uint32)]
+
+
+class _LoadOptions(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("use_amx", C.c_int32), ("prefix_dims", C.c_uint32),
+                ("batched_coarse", C.c_int32)]
+
+
+class _SearchParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("nprobe", C.c_uint32), ("ranking", C.c_int32),
+                ("survivors", C.c_uint32), ("selection", C.c_int32)]
+
+
+class _Extras(C.Structure):
+    _fields_ = [("nprobe_per_query", C.c_void_p), ("normalize", C.c_int32), ("device_io", C.c_int32),
+                ("probes", C.c_void_p), ("probe_dists", C.c_void_p), ("survivors", C.c_void_p),
+                ("n_survivors", C.c_void_p), ("rotated", C.c_void_p), ("codes", C.c_void_p),
+                ("stream", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libkatoann.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libkatoann.so is not built: run python katoann-public_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        P, u32, u64, i32, f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+        L.katoann_last_error.restype = C.c_char_p
+        L.katoann_index_create.argtypes = [C.POINTER(C.c_void_p)]
+        L.katoann_index_destroy.argtypes = [P]
+        L.katoann_load.argtypes = [P, C.c_char_p, C.POINTER(_LoadOptions)]
+        L.katoann_load_arrays.argtypes = [P, u32, u32, u32, i32, P, P, P, P, i32, P, u32, C.POINTER(_LoadOptions)]
+        L.katoann_search.argtypes = [P, P, u32, C.POINTER(_SearchParams), P, P]
+        L.katoann_search_ex.argtypes = [P, P, u32, C.POINTER(_SearchParams), C.POINTER(_Extras), P, P]
+        for name, rt in [("dim", u32), ("nlist", u32), ("size", u32), ("normalized", i32), ("engine", i32),
+                         ("precision", i32), ("prefix_dims", u32), ("rotated", i32), ("batched_coarse", i32),
+                         ("memory_bytes", u64)]:
+            f = getattr(L, "katoann_" + name)
+            f.restype = rt
+            f.argtypes = [P]
+        L.katoann_list_sizes.argtypes = [P, P]
+        L.katoann_index_scale.argtypes = [P, P, P, P]
+        L.katoann_export_codes.argtypes = [P, P, P, P]
+        L.katoann_set_timing.argtypes = [P, i32]
+        L.katoann_stage_times.argtypes = [P, P, i32]
+        L.katoann_last_work.argtypes = [P, P, i32]
+        L.katoann_set_rank_rows.argtypes = [P, u32]
+        L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
+        L.katoann_sq16_high_l2.argtypes = [P, P, P, P, f64, u32, u32, u32, P, u32]
+        L.katoann_normalize_rows.argtypes = [P, u64, u32]
+        L.katoann_rotate_rows.argtypes = [P, P, u64, u32, P]
+        L.katoann_build.argtypes = [P, u32, u32, C.POINTER(_BuildOptions), C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = lib().katoann_last_error().decode()
+    if rc == 1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(x, what="data must be a 2-D float32 array"):
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    if a.ndim != 2:
+        raise ValueError(what)
+    return a
+
+
+def amx_available() -> bool:
+    """The reference reports the CPU tile engine; this engine is the GPU."""
+    return False
+
+
+def build(data, prefix: str, nlist: int = 2048, train_size: int = 256000, kmeans_iters: int = 12,
+          seed: int = 0, normalized: bool = False) -> None:
+    """Partition data (n x dim float32) and write the index files under prefix
+    (python/katoann_module.cpp:39-58). The base data is not normalised here."""
+    data = _f32(data)
+    o = _BuildOptions(nlist, train_size, kmeans_iters, seed, int(normalized), 100000, 8)
+    _check(lib().katoann_build(_ptr(data), data.shape[0], data.shape[1], C.byref(o), prefix.encode()))
+
+
+class Index:
+    """katoann.Index (python/katoann_module.cpp:60-128)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().katoann_index_create(C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.katoann_index_destroy(h)
+            self._h = None
+
+    # ---- reference API
+    def load(self, prefix: str, precision: str = "sq16", use_amx: bool = True, prefix_dims: int = 0,
+             batched_coarse: bool = True) -> None:
+        if precision == "bf16":
+            prec = 1
+        elif precision == "sq16":
+            prec = 0
+        else:
+            raise ValueError("precision is 'sq16' or 'bf16'")
+        o = _LoadOptions(prec, 1 if use_amx else 0, prefix_dims, 1 if batched_coarse else 0)
+        _check(lib().katoann_load(self._h, prefix.encode(), C.byref(o)))
+
+    def search(self, queries, k: int = 10, nprobe: int = 64, ranking: str = "prefix", survivors: int = 32,
+               selection: str = "threshold"):
+        """Search all queries together; returns (ids, squared L2 distances), nearest first.
+        Queries are normalized when the index was built with normalized=True."""
+        return self.search_ex(queries, k=k, nprobe=nprobe, ranking=ranking, survivors=survivors,
+                              selection=selection)[:2]
+
+    @property
+    def dim(self):
+        return lib().katoann_dim(self._h)
+
+    @property
+    def nlist(self):
+        return lib().katoann_nlist(self._h)
+
+    @property
+    def size(self):
+        return lib().katoann_size(self._h)
+
+    @property
+    def normalized(self):
+        return bool(lib().katoann_normalized(self._h))
+
+    @property
+    def prefix_dims(self):
+        return lib().katoann_prefix_dims(self._h)
+
+    @property
+    def engine(self):
+        return {0: "amx", 1: "avx512", 2: "gfx950"}[lib().katoann_engine(self._h)]
+
+    @property
+    def memory_bytes(self):
+        return lib().katoann_memory_bytes(self._h)
+
+    # ---- extensions
+    @property
+    def rotated(self):
+        return bool(lib().katoann_rotated(self._h))
+
+    def list_sizes(self):
+        out = np.zeros(self.nlist, np.uint32)
+        _check(lib().katoann_list_sizes(self._h, _ptr(out)))
+        return out
+
+    def _params(self, k, nprobe, ranking, survivors, selection):
+        if ranking not in RANKING:
+            raise ValueError("ranking is 'prefix', 'high' or 'full'")
+        if selection not in SELECTION:
+            raise ValueError("selection is 'threshold' or 'local'")
+        return _SearchParams(k, nprobe, RANKING[ranking], survivors, SELECTION[selection])
+
+    def search_ex(self, queries, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold",
+                  nprobe_per_query=None, normalize=None, want=()):
+        """search() with optional per-query probe counts and extra outputs.
+        want: any of "probes", "probe_dists", "survivors", "rotated", "codes"."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim != 2 or q.shape[1] != self.dim:
+            raise ValueError("queries must be (n, dim) float32")
+        nq = q.shape[0]
+        p = self._params(k, nprobe, ranking, survivors, selection)
+        ids = np.empty((nq, k), np.uint32)
+        dists = np.empty((nq, k), np.float32)
+        x = _Extras()
+        x.normalize = int(self.normalized if normalize is None else normalize)
+        extra = {}
+        S = max(survivors, k)
+        npq = None
+        if nprobe_per_query is not None:
+            npq = np.ascontiguousarray(nprobe_per_query, dtype=np.uint32)
+            x.nprobe_per_query = _ptr(npq)
+        if "probes" in want:
+            extra["probes"] = np.empty((nq, nprobe), np.uint32)
+            x.probes = _ptr(extra["probes"])
+        if "probe_dists" in want:
+            extra["probe_dists"] = np.empty((nq, nprobe), np.float32)
+            x.probe_dists = _ptr(extra["probe_dists"])
+        if "survivors" in want:
+            extra["survivors"] = np.empty((nq, S), np.uint64)
+            extra["n_survivors"] = np.empty(nq, np.uint32)
+            x.survivors = _ptr(extra["survivors"])
+            x.n_survivors = _ptr(extra["n_survivors"])
+        if "rotated" in want:
+            extra["rotated"] = np.empty((nq, self.dim), np.float32)
+            x.rotated = _ptr(extra["rotated"])
+        if "codes" in want:
+            dc = (self.dim + 63) // 64 * 64
+            extra["codes"] = np.empty((nq, dc), np.uint16)
+            x.codes = _ptr(extra["codes"])
+        _check(lib().katoann_search_ex(self._h, _ptr(q), nq, C.byref(p), C.byref(x), _ptr(ids), _ptr(dists)))
+        return ids, dists, extra
+
+    def search_device(self, q_ptr: int, nq: int, ids_ptr: int, dists_ptr: int, k=10, nprobe=64,
+                      ranking="prefix", survivors=32, selection="threshold", npq_ptr: int | None = None,
+                      normalize=False, stream: int | None = None):
+        """Search with queries and outputs in device memory (e.g. torch tensors' data_ptr())."""
+        p = self._params(k, nprobe, ranking, survivors, selection)
+        x = _Extras()
+        x.device_io = 1
+        x.normalize = int(normalize)
+        x.nprobe_per_query = npq_ptr
+        x.stream = stream
+        _check(lib().katoann_search_ex(self._h, C.c_void_p(q_ptr), nq, C.byref(p), C.byref(x),
+                                       C.c_void_p(ids_ptr), C.c_void_p(dists_ptr)))
+
+    def load_arrays(self, dim, nlist, n, normalized, centroids, list_sizes, ids, vectors, rotation,
+                    prefix_dims, vectors_device_ptr: int | None = None):
+        """Load from arrays (vectors grouped by list, list order). With
+        vectors_device_ptr the vectors are read from device memory."""
+        cen = np.ascontiguousarray(centroids, np.float32)
+        sz = np.ascontiguousarray(list_sizes, np.uint32)
+        idv = np.ascontiguousarray(ids, np.uint32)
+        rot = None if rotation is None else np.ascontiguousarray(rotation, np.float32)
+        o = _LoadOptions(0, 1, 0, 1)
+        if vectors_device_ptr is not None:
+            vp, on_dev = C.c_void_p(vectors_device_ptr), 1
+        else:
+            vec = np.ascontiguousarray(vectors, np.float32)
+            vp, on_dev = _ptr(vec), 0
+        _check(lib().katoann_load_arrays(self._h, dim, nlist, n, int(normalized), _ptr(cen), _ptr(sz), _ptr(idv),
+                                         vp, on_dev, _ptr(rot), prefix_dims, C.byref(o)))
+
+    def scale(self):
+        lo, inv, ssq = C.c_float(), C.c_double(), C.c_double()
+        _check(lib().katoann_index_scale(self._h, C.byref(lo), C.byref(inv), C.byref(ssq)))
+        return lo.value, inv.value, ssq.value
+
+    def export_codes(self):
+        sizes = self.list_sizes()
+        n = int(sizes.sum())
+        dc = (self.dim + 63) // 64 * 64
+        codes = np.empty((n, dc), np.uint16)
+        cc = np.empty((self.nlist, dc), np.uint16)
+        ids = np.empty(n, np.uint32)
+        _check(lib().katoann_export_codes(self._h, _ptr(codes), _ptr(cc), _ptr(ids)))
+        return codes, cc, ids
+
+    def set_timing(self, on: bool = True):
+        _check(lib().katoann_set_timing(self._h, int(on)))
+
+    def stage_times(self):
+        out = np.zeros(NSTAGES, np.float32)
+        _check(lib().katoann_stage_times(self._h, _ptr(out), NSTAGES))
+        return dict(zip(STAGES, out.tolist()))
+
+    def last_work(self):
+        out = np.zeros(4, np.uint64)
+        _check(lib().katoann_last_work(self._h, _ptr(out), 4))
+        return dict(pairs=int(out[0]), items=int(out[1]))
+
+    def set_rank_rows(self, rows: int):
+        _check(lib().katoann_set_rank_rows(self._h, rows))
+
+
+# ---- kernel-level entry points (reference include/katoann/kernels.h:34-52)
+
+def sq16_l2(qhi, qlo, xhi, xlo, qnorm, xnorm, step_sq):
+    qhi, qlo, xhi, xlo = (np.ascontiguousarray(a, np.uint8) for a in (qhi, qlo, xhi, xlo))
+    qn, xn = np.ascontiguousarray(qnorm, np.int64), np.ascontiguousarray(xnorm, np.int64)
+    m, dc = qhi.shape
+    n = xhi.shape[0]
+    out = np.empty((m, n), np.float32)
+    _check(lib().katoann_sq16_l2(_ptr(qhi), _ptr(qlo), _ptr(xhi), _ptr(xlo), _ptr(qn), _ptr(xn), step_sq, m, n,
+                                 dc, _ptr(out)))
+    return out
+
+
+def sq16_high_l2(qhi, xhi, qnorm, xnorm, step_sq, padded_dim=None):
+    """qhi: m x padded_dim; xhi: n x plane_dim (the leading padded_dim bytes are read)."""
+    qhi, xhi = np.ascontiguousarray(qhi, np.uint8), np.ascontiguousarray(xhi, np.uint8)
+    qn, xn = np.ascontiguousarray(qnorm, np.int64), np.ascontiguousarray(xnorm, np.int64)
+    m, pd = qhi.shape
+    n, plane = xhi.shape
+    out = np.empty((m, n), np.float32)
+    _check(lib().katoann_sq16_high_l2(_ptr(qhi), _ptr(xhi), _ptr(qn), _ptr(xn), step_sq, m, n, pd, _ptr(out), plane))
+    return out
+
+
+def normalize_rows(x):
+    y = np.array(x, dtype=np.float32, copy=True, order="C")
+    _check(lib().katoann_normalize_rows(_ptr(y), y.shape[0], y.shape[1]))
+    return y
+
+
+def rotate_rows(x, rotation):
+    x = np.ascontiguousarray(x, np.float32)
+    r = np.ascontiguousarray(rotation, np.float32)
+    out = np.empty_like(x)
+    _check(lib().katoann_rotate_rows(_ptr(x), _ptr(out), x.shape[0], x.shape[1], _ptr(r)))
+    return out

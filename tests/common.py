@@ -1,0 +1,59 @@
+"""Shared helpers of the tests: fixtures on disk, comparison rules."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def same(a_ids, a_d, b_ids, b_d) -> bool:
+    """The reference's own equality (tests/test_index.cpp:43-52): identical
+    distances at every rank; ids equal wherever the distance is below the
+    query's k-th distance."""
+    if not np.array_equal(a_d.view(np.uint32), b_d.view(np.uint32)):
+        return False
+    last = a_d[:, -1:]
+    mask = a_d < last
+    return bool(np.array_equal(np.where(mask, a_ids, 0), np.where(mask, b_ids, 0)))
+
+
+def compare(ids, dists, ref_ids, ref_dists, tie_rows):
+    """Bit-exact on rows without ties; the reference's same() on tie rows.
+    Returns (n_exact_rows_checked, n_tie_rows)."""
+    tie = np.asarray(tie_rows, bool)
+    ok = ~tie
+    assert np.array_equal(ids[ok], ref_ids[ok]), "ids differ on %d tie-free rows" % int(
+        (ids[ok] != ref_ids[ok]).any(1).sum())
+    assert np.array_equal(dists[ok].view(np.uint32), ref_dists[ok].view(np.uint32)), "dists differ"
+    if tie.any():
+        assert same(ids[tie], dists[tie], ref_ids[tie], ref_dists[tie])
+    return int(ok.sum()), int(tie.sum())
+
+
+def golden_names():
+    return sorted(n for n in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, n)))
+
+
+def golden(name):
+    from katoann import formats
+    d = os.path.join(GOLDEN, name)
+    ref = np.load(os.path.join(d, "reference_outputs.npz"))
+    q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+    cases = []
+    for key in ref.files:
+        if key.startswith("ids_"):
+            k, nprobe, S, ranking, selection = key[4:].split("_")
+            cases.append((int(k), int(nprobe), int(S), ranking, selection))
+    return os.path.join(d, "idx"), q, ref, sorted(cases)
+
+
+def small_index(tmpdir, n=20000, nq=300, d=128, nlist=64, centres=64, seed=7, normalized=True, iters=5):
+    """A seeded mixture index written with the CPU builder; returns (prefix, base, queries)."""
+    from katoann import builder, datasets
+    base, q = datasets.mixture(n, nq, d=d, centres=centres, sigma=0.3, seed=seed)
+    prefix = os.path.join(str(tmpdir), "idx_%d_%d_%d" % (n, d, nlist))
+    builder.build_numpy(base, prefix, nlist=nlist, kmeans_iters=iters, seed=seed, normalized=normalized)
+    return prefix, base, q

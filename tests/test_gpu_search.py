@@ -1,0 +1,167 @@
+"""The GPU search against the reference (golden fixtures) and the oracle.
+
+Rule: bit-exact ids and distances on every query without a tie at a selection
+boundary (a coarse tie at the nprobe-th list, or, for threshold selection, a
+ranking tie at the S-th survivor), the reference's own same() on the others
+(tests/test_index.cpp:43-52). Ties are detected by the oracle.
+"""
+import numpy as np
+import pytest
+
+import common
+import katoann
+import oracle as O
+from katoann import formats
+
+pytestmark = pytest.mark.gpu
+
+
+def load(prefix, **kw):
+    ix = katoann.Index()
+    ix.load(prefix, **kw)
+    return ix
+
+
+@pytest.mark.parametrize("name", common.golden_names())
+def test_load_matches_reference(name):
+    prefix, q, ref, _ = common.golden(name)
+    print("golden load", name, flush=True)
+    ix = load(prefix)
+    lo, inv, ssq = ix.scale()
+    assert np.float32(lo) == ref["lo"] and inv == ref["inv_step"] and ssq == ref["step_sq"]
+    codes, ccodes, _ = ix.export_codes()
+    import hashlib
+    assert hashlib.sha256(codes.tobytes()).hexdigest() == str(ref["codes_sha"])
+    assert hashlib.sha256(ccodes.tobytes()).hexdigest() == str(ref["ccodes_sha"])
+    assert ix.prefix_dims == int(ref["prefix_dims"])
+
+
+@pytest.mark.parametrize("name", common.golden_names())
+def test_search_matches_reference(name):
+    prefix, q, ref, cases = common.golden(name)
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    rows = 0
+    for (k, nprobe, S, ranking, selection) in cases:
+        tag = f"{k}_{nprobe}_{S}_{ranking}_{selection}"
+        print("golden case", name, tag, flush=True)
+        ids, dists, _ = ix.search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, selection=selection,
+                                     normalize=False)
+        flags = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
+        tie = (flags & (1 if selection == "local" else 3)) != 0
+        n_ok, _ = common.compare(ids, dists, ref["ids_" + tag], ref["dists_" + tag], tie)
+        rows += n_ok
+    assert rows > 0
+
+
+@pytest.fixture(scope="module")
+def mixture_index(tmp_path_factory):
+    """The CPU-runnable workload's law at a reduced size: 256-centre mixture,
+    sigma 0.3, d 128, normalised; nlist 256."""
+    d = tmp_path_factory.mktemp("mix")
+    prefix, base, q = common.small_index(d, n=60000, nq=1000, d=128, nlist=256, centres=256, seed=1)
+    return prefix, base, q
+
+
+def check_against_oracle(ix, ox, q, k, nprobe, S, ranking="prefix", npq=None):
+    ids, dists, ex = ix.search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, normalize=False,
+                                  nprobe_per_query=npq, want=("probes", "probe_dists", "survivors"))
+    o = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, nprobe_per_query=npq)
+    coarse_tie = (o["flags"] & 1) != 0
+    ok = ~coarse_tie
+    # probes: same lists in the same (distance, list) order
+    assert np.array_equal(ex["probes"][ok], o["probes"][ok])
+    assert np.array_equal(ex["probe_dists"].view(np.uint32), o["probe_dists"].view(np.uint32))
+    # survivors: the S smallest (rank, id) keys
+    assert np.array_equal(ex["n_survivors"][ok], o["nsurv"][ok])
+    assert np.array_equal(ex["survivors"][ok], o["surv"][ok])
+    common.compare(ids, dists, o["ids"], o["dists"], coarse_tie)
+    return ids, dists, int(coarse_tie.sum())
+
+
+@pytest.mark.parametrize("k,nprobe,S", [(10, 16, 32), (10, 1, 10), (10, 64, 32), (100, 16, 100),
+                                        (5, 8, 128), (32, 256, 64)])
+def test_mixture_vs_oracle(mixture_index, k, nprobe, S):
+    prefix, base, q = mixture_index
+    print("mixture", k, nprobe, S, flush=True)
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    check_against_oracle(ix, ox, q, k, nprobe, S)
+
+
+def test_high_byte_ranking(mixture_index):
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    check_against_oracle(ix, ox, q[:300], 10, 8, 32, ranking="high")
+
+
+def test_per_query_nprobe(mixture_index):
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    rng = np.random.default_rng(3)
+    npq = rng.integers(1, 33, q.shape[0]).astype(np.uint32)
+    check_against_oracle(ix, ox, q, 10, 32, 32, npq=npq)
+
+
+def test_ragged_and_single_batches(mixture_index):
+    """Each query's result is the same alone, in a ragged batch and in the full batch."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    full_i, full_d, _ = ix.search_ex(q, k=10, nprobe=16, normalize=False)
+    for lo_, hi_ in ((0, 1), (5, 22), (100, 357), (999, 1000)):
+        i, d, _ = ix.search_ex(q[lo_:hi_], k=10, nprobe=16, normalize=False)
+        assert np.array_equal(i, full_i[lo_:hi_]) and np.array_equal(d, full_d[lo_:hi_])
+    i2, d2, _ = ix.search_ex(q, k=10, nprobe=16, normalize=False)
+    assert np.array_equal(i2, full_i) and np.array_equal(d2, full_d)
+
+
+def test_padding_when_candidates_run_out(tmp_path):
+    """Fewer candidates than k: FLT_MAX / UINT32_MAX padding (src/index.cpp:384-398)."""
+    prefix, base, q = common.small_index(tmp_path, n=300, nq=20, d=64, nlist=60, centres=40, seed=4)
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    ids, dists, _ = check_against_oracle(ix, ox, q, 40, 1, 40)
+    assert (ids == 0xFFFFFFFF).any() and (dists == np.finfo(np.float32).max).any()
+
+
+def test_python_search_normalises(mixture_index):
+    """Index.search normalises the queries of a normalised index on the GPU,
+    as the reference module does before Index::search."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ox = O.OracleIndex.from_files(prefix)
+    raw = (q[:200] * np.float32(3.5)).astype(np.float32)
+    ids, dists = ix.search(raw, k=10, nprobe=16)
+    o = ox.search(O.normalize_rows(raw), k=10, nprobe=16)
+    common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & 1) != 0)
+
+
+def test_argument_errors(mixture_index):
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    with pytest.raises(ValueError, match="need k >= 1"):
+        ix.search(q[:2], k=0)
+    with pytest.raises(ValueError, match="need k >= 1"):
+        ix.search(q[:2], nprobe=ix.nlist + 1)
+    with pytest.raises(ValueError, match="queries must be"):
+        ix.search(q[:2, :5])
+
+
+def test_build_load_search_end_to_end(tmp_path):
+    """katoann.build (host C++ builder) -> load -> search on the GPU, checked
+    against brute force recall and the oracle."""
+    from katoann import datasets
+    base, q = datasets.decaying_clusters(8000, 100, d=128, seed=11)
+    prefix = str(tmp_path / "e2e")
+    katoann.build(base, prefix, nlist=32, seed=1, normalized=True)
+    ix = load(prefix)
+    assert ix.rotated and ix.prefix_dims == 64 and ix.size == 8000
+    ids, dists = ix.search(q, k=10, nprobe=32, survivors=128)
+    gt = datasets.ground_truth_np(base, q, 10)
+    recall = np.mean([len(set(ids[i]) & set(gt[i])) / 10 for i in range(len(q))])
+    assert recall >= 0.95
+    ox = O.OracleIndex.from_files(prefix)
+    o = ox.search(q, k=10, nprobe=32, survivors=128)
+    common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & 1) != 0)
