@@ -55,8 +55,8 @@ def main():
     ap.add_argument("--nq", type=int, default=4096)
     ap.add_argument("--nlist", type=int, default=4096)
     ap.add_argument("--alpha", type=float, default=1.0)
-    ap.add_argument("--nprobe", default="16,32,64,128,256")
-    ap.add_argument("--survivors", type=int, default=32)
+    ap.add_argument("--nprobe", default="16,32,64,128,256,512,1024,2048")
+    ap.add_argument("--survivors", default="32,128,256")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--kmeans-iters", type=int, default=10)
     ap.add_argument("--rank-rows", type=int, default=0)
@@ -100,24 +100,26 @@ def main():
     gt = datasets.ground_truth_torch(base, queries, 10)
     log("ground truth %.1fs" % (time.time() - t0))
 
-    nq, k, S = args.nq, args.k, args.survivors
+    nq, k = args.nq, args.k
     ids_d = torch.empty((nq, k), dtype=torch.int32, device=dev)
     dists_d = torch.empty((nq, k), dtype=torch.float32, device=dev)
     qptr = queries.data_ptr()
 
-    def step(np_):
+    def step(np_, S):
         ix.search_device(qptr, nq, ids_d.data_ptr(), dists_d.data_ptr(), k=k, nprobe=np_, survivors=S)
 
     sweep = []
     ix.set_timing(True)
-    for np_ in [int(x) for x in args.nprobe.split(",")]:
+    grid = [(int(a), int(b)) for b in args.survivors.split(",") for a in args.nprobe.split(",")
+            if int(a) <= args.nlist]
+    for np_, S in grid:
         for _ in range(args.warmup):
-            step(np_)
+            step(np_, S)
         torch.cuda.synchronize()
         stage_sum = np.zeros(6)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(np_)
+            step(np_, S)
             st = ix.stage_times()
             stage_sum += np.array(list(st.values()))
         torch.cuda.synchronize()
@@ -130,9 +132,10 @@ def main():
         touched = np.unique(ex["probes"])
         rank_bytes = float(sizes[touched].astype(np.float64).sum() * (pd + 4))
         stages = dict(zip(katoann.STAGES, (stage_sum / args.steps).tolist()))
+        work = ix.last_work()
         rank_ms = stages["rank"]
-        row = dict(nprobe=np_, ms_per_step=ms, qps=nq / (ms / 1e3), recall=rec, stages_ms=stages,
-                   rank_bytes=rank_bytes, rank_gbs=rank_bytes / (rank_ms * 1e-3) / 1e9 if rank_ms > 0 else 0.0)
+        row = dict(nprobe=np_, survivors=S, ms_per_step=ms, qps=nq / (ms / 1e3), recall=rec, stages_ms=stages,
+                   work=work, rank_bytes=rank_bytes, rank_gbs=rank_bytes / (rank_ms * 1e-3) / 1e9 if rank_ms > 0 else 0.0)
         sweep.append(row)
         log(json.dumps(row))
     ix.set_timing(False)
@@ -147,7 +150,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(base, queries, b, sizes, pd, args, head["nprobe"])
+            cpu = cpu_baseline(base, queries, b, sizes, pd, args, head["nprobe"], head["survivors"])
         except Exception as e:  # reported, never silently replaced
             log("cpu baseline failed: %r" % (e,))
     out = {
@@ -163,22 +166,23 @@ def main():
         "vs_baseline": None,
         "dtype": "u8/int32 (16-bit codes, exact integer distances)",
         "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": "%dx%d i^-%g spectrum, nlist %d, prefix %d, S %d, k %d, batch %d" % (
-            args.n, args.dim, args.alpha, args.nlist, pd, S, k, nq),
-            "nprobe": head["nprobe"], "recall_at_10": head["recall"]},
+        "config": {"workload": "%dx%d i^-%g spectrum, nlist %d, prefix %d, k %d, batch %d" % (
+            args.n, args.dim, args.alpha, args.nlist, pd, k, nq),
+            "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"]},
         "qps_at_0999": b999["qps"] if b999 else None,
-        "nprobe_at_0999": b999["nprobe"] if b999 else None,
+        "at_0999": {"nprobe": b999["nprobe"], "survivors": b999["survivors"]} if b999 else None,
         "roofline": {"bound": "hbm", "kernel": "ranking pass", "achieved": head["rank_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["rank_gbs"] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": head["rank_bytes"], "ms_per_launch": head["stages_ms"]["rank"]},
         "stages_ms": head["stages_ms"],
-        "sweep": [{kk: r[kk] for kk in ("nprobe", "qps", "recall", "ms_per_step")} for r in sweep],
+        "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs")}
+                  for r in sweep],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(base, queries, b, sizes, pd, args, nprobe):
+def cpu_baseline(base, queries, b, sizes, pd, args, nprobe, survivors):
     """The reference's own CPU search (oracle/_ref/katoann, AVX-512 VNNI engine
     where AMX is absent) on this index and a sample of the same queries, timed by
     its CLI (one warm-up pass, median of 3) on this host's cores."""
@@ -191,7 +195,7 @@ def cpu_baseline(base, queries, b, sizes, pd, args, nprobe):
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         prefix = os.path.join(d, "cpu")
         order = b["order"].cpu().numpy().astype(np.uint32)
-        bounds = np.concatenate([[0], np.cumsum(sizes)])
+        bounds = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))]).astype(np.int64)
         host = base.cpu().numpy()
         lists = [order[bounds[c]:bounds[c + 1]] for c in range(len(sizes))]
         t0 = time.time()
@@ -200,7 +204,7 @@ def cpu_baseline(base, queries, b, sizes, pd, args, nprobe):
         formats.write_fbin(os.path.join(d, "q.fbin"), queries[:nq].cpu().numpy())
         log("cpu baseline: files written %.1fs" % (time.time() - t0))
         cmd = [O.REF_CLI, "search", "--index", prefix, "--queries", os.path.join(d, "q.fbin"), "--k", str(args.k),
-               "--nprobe", str(nprobe), "--survivors", str(args.survivors), "--batch", "4096", "--engine", "avx512",
+               "--nprobe", str(nprobe), "--survivors", str(survivors), "--batch", "4096", "--engine", "avx512",
                "--threads", str(threads)]
         t0 = time.time()
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
@@ -215,8 +219,9 @@ def cpu_baseline(base, queries, b, sizes, pd, args, nprobe):
     except Exception:
         model = "unknown"
     return {"value": res["qps"], "unit": "queries/s", "cores": threads, "kind": "reference",
-            "sample": "%d of the %d queries, nprobe %d, batch %d, median of 3 after a warm-up (reference CLI); %s" % (
-                nq, queries.shape[0], nprobe, min(nq, 4096), model),
+            "sample": "%d of the %d queries, nprobe %d, survivors %d, batch %d, median of 3 after a warm-up "
+                      "(reference CLI, AVX-512 VNNI engine); %s" % (nq, queries.shape[0], nprobe, survivors,
+                                                                     min(nq, 4096), model),
             "load_s": hdr["load_s"]}
 
 

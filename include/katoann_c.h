@@ -142,7 +142,7 @@ int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *cent
 int katoann_set_timing(katoann_index *h, int32_t on);
 int katoann_stage_times(const katoann_index *h, float *ms, int32_t n);
 /* Work of the last search: [0] (query, list) pairs, [1] ranking items,
- * [2] candidate pairs (query x row), [3] pool entries written. */
+ * [2] candidates kept by the ranking filter, [3] probe-rank bands. */
 int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n);
 
 /* Tile parameters of the ranking pass: rows per work item (multiple of 64). */

@@ -94,12 +94,63 @@ __device__ inline uint64_t wave_bitonic_merge64(uint64_t v) {
   return v;
 }
 
-// Top list of up to SC keys per query, SC = 64 (one per lane) or 128 (two).
-// e[0] holds indices 0..63, e[1] indices 64..127. Empty slots are kNoKey.
+// Sorted list of up to SC keys, SC = 64 * R (R a power of two): register r
+// of lane l holds entry 64 r + l. Empty slots are kNoKey.
 template <int SC>
 struct TopList {
   uint64_t e[SC / 64];
 };
+
+// Bitonic merge of a bitonic sequence of SC entries into ascending order:
+// strides >= 64 compare registers, strides < 64 compare lanes.
+template <int SC>
+__device__ inline void list_bitonic_merge(TopList<SC> &t) {
+  constexpr int R = SC / 64;
+#pragma unroll
+  for (int j = R / 2; j >= 1; j >>= 1)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if ((r & j) == 0) {
+        const uint64_t lo = umin64(t.e[r], t.e[r + j]), hi = umax64(t.e[r], t.e[r + j]);
+        t.e[r] = lo;
+        t.e[r + j] = hi;
+      }
+#pragma unroll
+  for (int r = 0; r < R; ++r) t.e[r] = wave_bitonic_merge64(t.e[r]);
+}
+
+// Full ascending sort of the SC entries: the bitonic network over index
+// i = 64 r + lane (stride >= 64: registers; stride < 64: lanes).
+template <int SC>
+__device__ inline void list_sort(TopList<SC> &t) {
+  constexpr int R = SC / 64;
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= SC; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const int jr = j / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if ((r & jr) == 0) {
+            const int p = r | jr;
+            const bool up = ((64 * r) & k) == 0;
+            const uint64_t lo = umin64(t.e[r], t.e[p]), hi = umax64(t.e[r], t.e[p]);
+            t.e[r] = up ? lo : hi;
+            t.e[p] = up ? hi : lo;
+          }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const bool up = ((64 * r + lane) & k) == 0;
+          const bool lower = (lane & j) == 0;
+          t.e[r] = cmpx(t.e[r], j, lower == up);
+        }
+      }
+    }
+  }
+}
 
 // Keeps the S smallest of (list, c), where c holds at most one key per lane
 // (kNoKey where none). Keys are unique.
@@ -108,15 +159,8 @@ __device__ inline void top_merge(TopList<SC> &t, uint64_t c, uint32_t S) {
   const int lane = lane_id();
   const uint64_t cs = wave_sort64(c);
   const uint64_t crev = __shfl(cs, 63 - lane);
-  if (SC == 64) {
-    t.e[0] = wave_bitonic_merge64(umin64(t.e[0], crev));
-  } else {
-    uint64_t m0 = t.e[0];
-    uint64_t m1 = umin64(t.e[SC / 64 - 1], crev);
-    const uint64_t lo = umin64(m0, m1), hi = umax64(m0, m1);
-    t.e[0] = wave_bitonic_merge64(lo);
-    t.e[SC / 64 - 1] = wave_bitonic_merge64(hi);
-  }
+  t.e[SC / 64 - 1] = umin64(t.e[SC / 64 - 1], crev);
+  list_bitonic_merge<SC>(t);
 #pragma unroll
   for (int r = 0; r < SC / 64; ++r)
     if (static_cast<uint32_t>(lane + 64 * r) >= S) t.e[r] = kNoKey;
@@ -162,6 +206,47 @@ __device__ __forceinline__ uint32_t top_count(const TopList<SC> &t) {
 #pragma unroll
   for (int r = 0; r < SC / 64; ++r) n += __popcll(ballot(t.e[r] != kNoKey));
   return n;
+}
+
+// The S-th smallest (1-based) of the valid keys held CH per lane (kNoKey
+// slots ignored); at least S valid keys. Bitwise search on the float bits,
+// then on the ids among keys sharing the float of the S-th.
+template <int CH>
+__device__ inline uint64_t kth_key(const uint64_t (&e)[CH], uint32_t S) {
+  uint32_t hi = 0;
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t t = hi | (1u << b);
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) c += __popcll(ballot(static_cast<uint32_t>(e[i] >> 32) < t));
+    if (c < S) hi = t;
+  }
+  uint32_t lt = 0, eq = 0;
+  uint32_t mlo = 0;
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const uint32_t h = static_cast<uint32_t>(e[i] >> 32);
+    lt += __popcll(ballot(h < hi));
+    eq += __popcll(ballot(h == hi));
+    if (h == hi) mlo = max(mlo, static_cast<uint32_t>(e[i]));
+  }
+  const uint32_t need = S - lt;
+  uint32_t lo = 0;
+  if (need == eq) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mlo = max(mlo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mlo), o)));
+    lo = mlo;
+  } else {
+    for (int b = 31; b >= 0; --b) {
+      const uint32_t t = lo | (1u << b);
+      uint32_t c = 0;
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        c += __popcll(ballot(static_cast<uint32_t>(e[i] >> 32) == hi && static_cast<uint32_t>(e[i]) < t));
+      if (c < need) lo = t;
+    }
+  }
+  return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 }  // namespace katoann_dev

@@ -31,7 +31,8 @@ constexpr uint32_t kMaxDim = 4096;          // kernels.h:24
 constexpr uint32_t kDimcMagic = 0x434D4944u;  // index.cpp:27
 constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
 constexpr uint32_t kQueriesPerItem = 64;    // ranking kernel's query chunk
-constexpr uint32_t kMaxSurvivors = 128;
+constexpr uint32_t kMaxSurvivors = 1024;
+constexpr uint32_t kPoolAllowance = 1024;  // pool slots per query beyond S for one band's kept keys
 constexpr uint32_t kMaxProbes = 4096;
 
 thread_local std::string g_last_error;
@@ -82,7 +83,7 @@ struct SearchCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[KATOANN_NSTAGES + 1] = {};
   DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, gfill, item_off, qlist,
-      totals, pool_off, pool_cnt, thr, state, pool, ids, dists, surv, nsurv, codes;
+      totals, pool_cnt, pool_sorted, thr, state, pool, ids, dists, surv, nsurv, codes, items;
   std::vector<uint32_t> host_totals;
   SearchCtx() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -153,16 +154,15 @@ struct katoann_index {
     self->free_ctx.push_back(std::move(c));
   }
 
-  // Upper bound of the ranking items any one query takes part in at nprobe np.
-  uint64_t max_items_per_query(uint32_t np) {
+  // Upper bound of the candidates of any one query at nprobe np: the sizes of
+  // the np largest lists.
+  uint64_t max_rows_per_query(uint32_t np) {
     std::lock_guard<std::mutex> g(mu);
-    if (tile_prefix_rows != rank_rows || tile_prefix.size() != nlist + 1u) {
-      std::vector<uint64_t> t(nlist);
-      for (uint32_t c = 0; c < nlist; ++c) t[c] = (sizes[c] + rank_rows - 1) / rank_rows;
+    if (tile_prefix.size() != nlist + 1u) {
+      std::vector<uint64_t> t(sizes.begin(), sizes.end());
       std::sort(t.begin(), t.end(), std::greater<uint64_t>());
       tile_prefix.assign(nlist + 1, 0);
       for (uint32_t c = 0; c < nlist; ++c) tile_prefix[c + 1] = tile_prefix[c] + t[c];
-      tile_prefix_rows = rank_rows;
     }
     return tile_prefix[std::min(np, nlist)];
   }
@@ -411,7 +411,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     throw std::invalid_argument("selection is 'threshold' or 'local'");
   const uint32_t S = std::max(p.survivors, p.k);
   if (S > kMaxSurvivors)
-    throw std::invalid_argument("max(survivors, k) above 128 is not supported by the gfx950 engine yet");
+    throw std::invalid_argument("max(survivors, k) above 1024 is not supported by the gfx950 engine");
   if (p.nprobe > kMaxProbes) throw std::invalid_argument("nprobe above 4096 is not supported by the gfx950 engine");
   const bool prefix = p.ranking == KATOANN_RANK_PREFIX;
   const uint32_t P = prefix ? ix.head_dims : ix.dc;
@@ -435,19 +435,29 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.dist.ensure(static_cast<size_t>(B) * nlist * 4);
   c.probes.ensure(static_cast<size_t>(B) * np * 4);
   c.probe_d.ensure(static_cast<size_t>(B) * np * 4);
-  c.gcnt.ensure(nlist * 4ull);
-  c.qoff.ensure(nlist * 4ull);
-  c.gfill.ensure(nlist * 4ull);
-  c.item_off.ensure((nlist + 1ull) * 4);
+  const uint32_t nslots = katoann_hip::plan_slots(nlist, np);
+  c.gcnt.ensure(nslots * 4ull);
+  c.qoff.ensure(nslots * 4ull);
+  c.gfill.ensure(nslots * 4ull);
+  c.item_off.ensure((nslots + 1ull) * 4);
   c.qlist.ensure(static_cast<size_t>(B) * np * 4);
   c.totals.ensure(16);
-  c.pool_off.ensure((B + 1ull) * 8);
   c.pool_cnt.ensure(B * 4ull);
+  c.pool_sorted.ensure(B * 4ull);
   c.thr.ensure(B * 8ull);
-  c.state.ensure(16);
+  const uint32_t nbands = katoann_hip::plan_bands(np);
+  const uint32_t nstate = 3 + nbands;
+  c.state.ensure(nstate * 4ull);
   const uint32_t rt = ix.rank_rows;
-  const uint64_t cap = static_cast<uint64_t>(S) * B * mix.max_items_per_query(np);
-  c.pool.ensure(std::max<uint64_t>(cap, 1) * 8);
+  const uint64_t max_rows = mix.max_rows_per_query(np);
+  // items <= sum over lists of row tiles x (ceil(B / 64) + bands): a list's query
+  // groups over all bands hold at most B queries
+  uint64_t tiles_total = 0;
+  for (uint32_t cc = 0; cc < nlist; ++cc) tiles_total += (ix.sizes[cc] + rt - 1) / rt;
+  const uint64_t item_cap64 = tiles_total * ((B + kQueriesPerItem - 1) / kQueriesPerItem + nbands);
+  if (item_cap64 > 0x7fffffffull) throw std::invalid_argument("batch too large for the ranking plan");
+  const uint32_t item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
+  c.items.ensure(static_cast<size_t>(item_cap) * 16);
   c.ids.ensure(static_cast<size_t>(B) * K * 4);
   c.dists.ensure(static_cast<size_t>(B) * K * 4);
   if (x && x->survivors) c.surv.ensure(static_cast<size_t>(B) * S * 8);
@@ -463,53 +473,81 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   auto mark = [&](int i) {
     if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
   };
-
-  HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
-                          dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-  mark(0);
-  if (x && x->normalize) katoann_hip::launch_normalize(c.q.as<float>(), B, d, s);
-  const float *yq = c.q.as<float>();
-  if (ix.rotated) {
-    katoann_hip::launch_rotate(c.q.as<float>(), B, d, ix.rot.as<float>(), c.y.as<float>(), s);
-    yq = c.y.as<float>();
-  }
-  katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
-                               c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(), c.qh0.as<int32_t>(),
-                               nullptr, (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
-  mark(1);
-  katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
-                             ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist, dc,
-                             ix.step_sq, c.dist.as<float>(), s);
-  mark(2);
-  katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(), s);
-  mark(3);
-  katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
-                           kQueriesPerItem, rt, S, c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(),
-                           c.item_off.as<uint32_t>(), c.gfill.as<uint32_t>(), c.qlist.as<uint32_t>(),
-                           c.totals.as<uint32_t>(), c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(),
-                           c.thr.as<uint64_t>(), c.state.as<uint32_t>(), s);
-  mark(4);
-  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 2;
-  if (!katoann_hip::launch_rank(prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>(),
-                                prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>(), ix.ids.as<uint32_t>(),
-                                ix.list_off_d.as<uint64_t>(), nlist, P, c.qhi.as<int8_t>(), dc,
-                                prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>(), c.gcnt.as<uint32_t>(),
-                                c.qoff.as<uint32_t>(), c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                                c.totals.as<uint32_t>(), rt, S, ix.step_sq, c.pool.as<uint64_t>(),
-                                c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
-                                c.state.as<uint32_t>(), grid, s))
-    throw std::invalid_argument("unsupported ranking shape");
-  mark(5);
+  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 4;
   uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
   float *ddst = dev_io ? out_dists : c.dists.as<float>();
-  katoann_hip::launch_refine(c.pool.as<uint64_t>(), c.pool_off.as<uint64_t>(), c.pool_cnt.as<uint32_t>(),
-                             c.thr.as<uint64_t>(), B, S, K, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(),
-                             ix.norm.as<int64_t>(), ix.id_row.as<uint32_t>(), ix.npts, dc, c.qhi.as<int8_t>(),
-                             c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), ix.step_sq, dids, ddst,
-                             (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
-                             (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s);
-  mark(6);
-  HIPCHECK(hipGetLastError());
+  const float *yq = c.q.as<float>();
+  uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+
+  // pool: S slots plus one band's kept keys per query; on overflow the batch
+  // runs again with room for every candidate of the query
+  auto run = [&](uint64_t allowance) {
+    const uint32_t stride = static_cast<uint32_t>(std::min<uint64_t>(S + allowance, 0xffffffffu));
+    c.pool.ensure(std::max<uint64_t>(static_cast<uint64_t>(stride) * B, 1) * 8);
+    HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
+                            dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    mark(0);
+    if (x && x->normalize) katoann_hip::launch_normalize(c.q.as<float>(), B, d, s);
+    yq = c.q.as<float>();
+    if (ix.rotated) {
+      katoann_hip::launch_rotate(c.q.as<float>(), B, d, ix.rot.as<float>(), c.y.as<float>(), s);
+      yq = c.y.as<float>();
+    }
+    katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
+                                 c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(),
+                                 c.qh0.as<int32_t>(), nullptr, (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
+    mark(1);
+    katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
+                               ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist,
+                               dc, ix.step_sq, c.dist.as<float>(), s);
+    mark(2);
+    katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
+                               s);
+    mark(3);
+    katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist, kQueriesPerItem,
+                             rt, c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
+                             c.gfill.as<uint32_t>(), c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
+                             c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(), c.thr.as<uint64_t>(),
+                             c.state.as<uint32_t>(), nstate, s);
+    katoann_hip::launch_items(c.item_off.as<uint32_t>(), nslots, nlist, ix.list_off_d.as<uint64_t>(),
+                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), rt, item_cap, c.items.as<uint4>(), s);
+    mark(4);
+    const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
+    const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
+    const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
+    katoann_hip::launch_seed(plane, xnorm, ix.ids.as<uint32_t>(), ix.list_off_d.as<uint64_t>(), P,
+                             c.qhi.as<int8_t>(), dc, qn, c.probes.as<uint32_t>(), npq, np, B, S, ix.step_sq,
+                             c.thr.as<uint64_t>(), s);
+    for (uint32_t band = 0; band < nbands; ++band) {
+      if (!katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, c.qhi.as<int8_t>(),
+                                         dc, qn, c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
+                                         c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
+                                         c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(), c.state.as<uint32_t>(), grid,
+                                         s))
+        throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+      katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
+                                 c.thr.as<uint64_t>(), B, S, stride, s);
+    }
+    mark(5);
+    if (!katoann_hip::launch_refine(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
+                                    B, S, K, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
+                                    ix.id_row.as<uint32_t>(), ix.npts, dc, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(),
+                                    c.qnorm.as<int64_t>(), ix.step_sq, dids, ddst,
+                                    (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
+                                    (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s))
+      throw std::invalid_argument("unsupported refinement shape");
+    mark(6);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(st, c.state.p, 12, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(st + 4, c.totals.p, 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return st[0] == 0;
+  };
+  if (!run(kPoolAllowance) && !run(max_rows)) {
+    ix.release(std::move(ctx));
+    throw std::runtime_error("survivor pool overflow in the ranking pass");
+  }
+
   const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (!dev_io) {
     HIPCHECK(hipMemcpyAsync(out_ids, dids, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
@@ -524,9 +562,6 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     if (x->rotated) HIPCHECK(hipMemcpyAsync(x->rotated, yq, static_cast<size_t>(B) * d * 4, d2o, s));
     if (x->codes) HIPCHECK(hipMemcpyAsync(x->codes, c.codes.p, static_cast<size_t>(B) * dc * 2, d2o, s));
   }
-  uint32_t st[4] = {0, 0, 0, 0};
-  HIPCHECK(hipMemcpyAsync(st, c.state.p, 8, hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipMemcpyAsync(st + 2, c.totals.p, 8, hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
   if (x && npq && !dev_io) {  // past a query's own count: no list
     for (uint32_t q = 0; q < B; ++q)
@@ -543,12 +578,12 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
         (void)hipEventElapsedTime(&ms, c.ev[i], c.ev[i + 1]);
         mix.stage_ms[i] = ms;
       }
-    mix.work[0] = st[2];
-    mix.work[1] = st[3];
+    mix.work[0] = st[4];  // (query, list) pairs
+    mix.work[1] = st[5];  // ranking items
+    mix.work[2] = st[1];  // candidates kept
+    mix.work[3] = nbands;
   }
-  const bool overflow = st[1] != 0;
   ix.release(std::move(ctx));
-  if (overflow) throw std::runtime_error("survivor pool overflow in the ranking pass");
 }
 
 // Row sums of shifted bytes for the kernel-level API's exact unsigned products.

@@ -396,13 +396,18 @@ __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Work is grouped by (probe-rank band, list): band b holds ranks [2^b - 1, 2^(b+1) - 1),
+// so a query's nearest lists come first in the queue and its threshold is
+// tight when its farther lists are ranked.
+__device__ __forceinline__ uint32_t band_of(uint32_t p) { return 31u - __clz(p + 1u); }
+
 __global__ void k_hist(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                       uint32_t np, uint32_t *gcnt) {
+                       uint32_t np, uint32_t nlist, uint32_t *gcnt) {
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
   if (npq && p >= npq[q]) return;
-  atomicAdd(&gcnt[probes[t]], 1u);
+  atomicAdd(&gcnt[band_of(p) * nlist + probes[t]], 1u);
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -434,15 +439,15 @@ __device__ inline uint64_t block_exscan(uint64_t v, uint64_t *tmp, uint64_t *tot
 
 __global__ __launch_bounds__(1024) void k_plan_lists(const uint32_t *__restrict__ gcnt,
                                                      const uint64_t *__restrict__ list_off, uint32_t nlist,
-                                                     uint32_t qc, uint32_t rt, uint32_t *qoff, uint32_t *item_off,
-                                                     uint32_t *gfill, uint32_t *totals) {
+                                                     uint32_t nslots, uint32_t qc, uint32_t rt, uint32_t *qoff,
+                                                     uint32_t *item_off, uint32_t *gfill, uint32_t *totals) {
   __shared__ uint64_t tmp[17];
-  const uint32_t per = (nlist + 1023) / 1024;
-  const uint32_t b = threadIdx.x * per, e = min(nlist, b + per);
+  const uint32_t per = (nslots + 1023) / 1024;
+  const uint32_t b = threadIdx.x * per, e = min(nslots, b + per);
   uint64_t sg = 0, si = 0;
   for (uint32_t c = b; c < e; ++c) {
     const uint32_t g = gcnt[c];
-    const uint64_t n = list_off[c + 1] - list_off[c];
+    const uint64_t n = list_off[c % nlist + 1] - list_off[c % nlist];
     sg += g;
     si += (g && n) ? static_cast<uint64_t>((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0;
   }
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(1024) void k_plan_lists(const uint32_t *__restrict_
   uint64_t oi = block_exscan(si, tmp, &ti);
   for (uint32_t c = b; c < e; ++c) {
     const uint32_t g = gcnt[c];
-    const uint64_t n = list_off[c + 1] - list_off[c];
+    const uint64_t n = list_off[c % nlist + 1] - list_off[c % nlist];
     qoff[c] = static_cast<uint32_t>(og);
     item_off[c] = static_cast<uint32_t>(oi);
     gfill[c] = 0;
@@ -459,448 +464,30 @@ __global__ __launch_bounds__(1024) void k_plan_lists(const uint32_t *__restrict_
     oi += (g && n) ? static_cast<uint64_t>((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0;
   }
   if (threadIdx.x == 0) {
-    item_off[nlist] = static_cast<uint32_t>(ti);
+    item_off[nslots] = static_cast<uint32_t>(ti);
     totals[0] = static_cast<uint32_t>(tg);
     totals[1] = static_cast<uint32_t>(ti);
   }
 }
 
 __global__ void k_scatter(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                          uint32_t np, const uint32_t *__restrict__ qoff, uint32_t *gfill, uint32_t *qlist) {
+                          uint32_t np, uint32_t nlist, const uint32_t *__restrict__ qoff, uint32_t *gfill,
+                          uint32_t *qlist) {
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
   if (npq && p >= npq[q]) return;
-  const uint32_t c = probes[t];
+  const uint32_t c = band_of(p) * nlist + probes[t];
   qlist[qoff[c] + atomicAdd(&gfill[c], 1u)] = q;
 }
 
-__global__ __launch_bounds__(1024) void k_plan_queries(const uint32_t *__restrict__ probes,
-                                                       const uint32_t *__restrict__ npq, uint32_t B, uint32_t np,
-                                                       const uint64_t *__restrict__ list_off, uint32_t rt,
-                                                       uint32_t S, uint64_t *pool_off, uint32_t *pool_cnt,
-                                                       uint64_t *thr, uint32_t *state) {
-  __shared__ uint64_t tmp[17];
-  const uint32_t per = (B + 1023) / 1024;
-  const uint32_t b = threadIdx.x * per, e = min(B, b + per);
-  uint64_t s = 0;
-  for (uint32_t q = b; q < e; ++q) {
-    const uint32_t m = npq ? min(npq[q], np) : np;
-    for (uint32_t p = 0; p < m; ++p) {
-      const uint32_t c = probes[static_cast<size_t>(q) * np + p];
-      const uint64_t n = list_off[c + 1] - list_off[c];
-      s += (n + rt - 1) / rt;
-    }
-  }
-  uint64_t total = 0;
-  uint64_t o = block_exscan(s, tmp, &total);
-  for (uint32_t q = b; q < e; ++q) {
-    pool_off[q] = o * S;
-    const uint32_t m = npq ? min(npq[q], np) : np;
-    for (uint32_t p = 0; p < m; ++p) {
-      const uint32_t c = probes[static_cast<size_t>(q) * np + p];
-      const uint64_t n = list_off[c + 1] - list_off[c];
-      o += (n + rt - 1) / rt;
-    }
-    pool_cnt[q] = 0;
-    thr[q] = kNoKey;
-  }
-  if (threadIdx.x == 0) {
-    pool_off[B] = total * S;
-    state[0] = 0;  // work queue head
-    state[1] = 0;  // pool overflow flag
-  }
-}
-
-// --------------------------------------------------------------------------
-// Ranking pass (src/index.cpp:998-1044 with kernels.cpp:296-316): for every
-// (query, candidate) of the probed lists, r = f(qn + xn - 2 sum_{k<P} qh xh)
-// over the high bytes of the leading P rotated dims, keeping each query's S
-// smallest (r, id) keys.
-//
-// Persistent: each 256-thread block takes items (list, up to 64 queries, row
-// tile) from a device counter until the queue is empty. Rows of the tile are
-// staged from HBM into LDS 64 at a time, once for all queries of the item;
-// each wave owns 16 queries (MFMA A operand in registers) and keeps their
-// top lists in LDS. A candidate reaches a top list only if its integer l2 is
-// within the bound of the query's current S-th key (its own list or the
-// global per-query threshold). At the end of the item each query's list goes
-// to the query's pool (only keys at or below the global threshold), and the
-// threshold drops to the list's S-th key. The pool holds a superset of the
-// final S smallest keys, so the final selection is exact whatever the timing.
-struct RankArgs {
-  const int8_t *plane;       // N x P, row-major, shifted high bytes
-  const int32_t *xnorm;      // N: shifted norm of those bytes
-  const uint32_t *ids;       // N
-  const uint64_t *list_off;  // nlist + 1
-  uint32_t nlist;
-  uint32_t P;
-  const int8_t *qhi;         // B x dc
-  uint32_t dc;
-  const int32_t *qnorm;      // B: shifted norm of the query's leading P high bytes
-  const uint32_t *gcnt, *qoff, *qlist, *item_off, *totals;
-  uint32_t rt;               // rows per tile
-  uint32_t S;
-  int32_t lmax;              // P * 255^2
-  double step_sq;
-  uint64_t *pool;
-  const uint64_t *pool_off;  // B + 1
-  uint32_t *pool_cnt;
-  uint64_t *thr;
-  uint32_t *state;           // [0] queue head, [1] overflow
-};
-
-constexpr int RK_QC = 64;   // queries per item (16 per wave)
-constexpr int RK_ROWS = 64; // rows per LDS stage
-
-template <int KC, int SC>
-__global__ __launch_bounds__(256) void k_rank(RankArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  int8_t *rows = reinterpret_cast<int8_t *>(smem);                                   // 4 x KC x 1 KB
-  int32_t *T = reinterpret_cast<int32_t *>(smem + 4 * KC * 1024);                     // 4 x 16 x 64
-  uint64_t *E = reinterpret_cast<uint64_t *>(smem + 4 * KC * 1024 + 4 * 16 * 64 * 4); // 64 x SC
-  __shared__ uint32_t s_item[6];
-  __shared__ uint32_t s_qid[RK_QC];
-  __shared__ int32_t s_qn[RK_QC];
-  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-  constexpr uint32_t P = KC * 64;
-  const uint32_t total_items = a.totals[1];
-
-  for (;;) {
-    __syncthreads();
-    if (tid == 0) {
-      const uint32_t it = atomicAdd(&a.state[0], 1u);
-      s_item[0] = it;
-      if (it < total_items) {
-        uint32_t lo = 0, hi = a.nlist;  // item_off[lo] <= it < item_off[hi]
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) / 2;
-          if (a.item_off[mid] <= it) lo = mid;
-          else hi = mid;
-        }
-        const uint32_t c = lo;
-        const uint64_t n = a.list_off[c + 1] - a.list_off[c];
-        const uint32_t tiles = static_cast<uint32_t>((n + a.rt - 1) / a.rt);
-        const uint32_t local = it - a.item_off[c];
-        const uint32_t qch = local / tiles, rtile = local % tiles;
-        const uint32_t g = a.gcnt[c];
-        s_item[1] = a.qoff[c] + qch * RK_QC;                          // first query slot
-        s_item[2] = min(static_cast<uint32_t>(RK_QC), g - qch * RK_QC); // queries
-        s_item[3] = static_cast<uint32_t>(a.list_off[c] + static_cast<uint64_t>(rtile) * a.rt);
-        s_item[4] = static_cast<uint32_t>(a.list_off[c] + min(n, static_cast<uint64_t>(rtile + 1) * a.rt));
-      }
-    }
-    __syncthreads();
-    if (s_item[0] >= total_items) break;
-    const uint32_t qbeg = s_item[1], nq = s_item[2], rbeg = s_item[3], rend = s_item[4];
-    if (tid < RK_QC) {
-      const uint32_t qid = tid < static_cast<int>(nq) ? a.qlist[qbeg + tid] : 0u;
-      s_qid[tid] = qid;
-      s_qn[tid] = tid < static_cast<int>(nq) ? a.qnorm[qid] : 0;
-    }
-    for (int i = tid; i < RK_QC * SC; i += 256) E[i] = kNoKey;
-    __syncthreads();
-    const int wq = max(0, min(16, static_cast<int>(nq) - 16 * w));  // queries of this wave
-
-    // A operand: the wave's 16 queries, k-slice lane / 16 of each 64-byte chunk
-    v4i areg[KC];
-    {
-      const int sl = 16 * w + (lane & 15);
-      const bool ok = (lane & 15) < wq;
-      const int8_t *qp = a.qhi + static_cast<size_t>(s_qid[ok ? sl : 0]) * a.dc + 16 * (lane >> 4);
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) areg[kc] = ok ? load16(qp + 64 * kc) : v4i{0, 0, 0, 0};
-    }
-
-    for (uint32_t r0 = rbeg; r0 < rend; r0 += RK_ROWS) {
-      __syncthreads();
-      // stage 64 rows x P bytes, MFMA fragment order: [row/16][kc][lane][16 B]
-      constexpr int PIECES = P / 16;
-      for (int e = tid; e < RK_ROWS * PIECES; e += 256) {
-        const int rl = e / PIECES, pc = e % PIECES;
-        const uint32_t gr = min(r0 + rl, rend - 1);
-        const v4i v = load16(a.plane + static_cast<size_t>(gr) * P + 16 * pc);
-        const int off = (((rl >> 4) * KC + (pc >> 2)) * 64 + (rl & 15) + 16 * (pc & 3)) * 16;
-        *reinterpret_cast<v4i *>(rows + off) = v;
-      }
-      __syncthreads();
-      // per-query l2 bound from the global threshold and the query's own list
-      int32_t Lv = -1;
-      if (lane < wq) {
-        const int slot = 16 * w + lane;
-        const uint64_t g = __atomic_load_n(&a.thr[s_qid[slot]], __ATOMIC_RELAXED);
-        const uint64_t ks = E[slot * SC + a.S - 1];
-        const uint64_t t = umin64(g, ks);
-        Lv = t == kNoKey ? 0x7fffffff
-                         : static_cast<int32_t>(l2_bound(key_float(t), a.step_sq, a.lmax));
-      }
-      if (wq > 0) {
-        int32_t *Tw = T + w * 16 * 64;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          v4i acc = {0, 0, 0, 0};
-#pragma unroll
-          for (int kc = 0; kc < KC; ++kc) {
-            const v4i b = *reinterpret_cast<const v4i *>(rows + ((t * KC + kc) * 64 + lane) * 16);
-            acc = mfma(areg[kc], b, acc);
-          }
-          const uint32_t rr = 16 * t + (lane & 15);
-          const uint32_t gr = r0 + rr;
-          const bool rok = gr < rend;
-          const int32_t xn = rok ? a.xnorm[gr] : 0;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int sq = 4 * (lane >> 4) + r;
-            const bool ok = rok && sq < wq;
-            Tw[sq * 64 + rr] = ok ? s_qn[16 * w + sq] + xn - 2 * acc[r] : -1;
-          }
-        }
-      }
-      __syncthreads();
-      if (wq > 0) {
-        const int32_t *Tw = T + w * 16 * 64;
-        for (int sq = 0; sq < wq; ++sq) {
-          const int32_t v = Tw[sq * 64 + lane];
-          const int32_t L = __shfl(Lv, sq);
-          const bool pass = v >= 0 && v <= L;
-          const uint64_t m = ballot(pass);
-          if (!m) continue;
-          const uint32_t gr = min(r0 + lane, rend - 1);
-          const uint64_t key = pass ? make_key(fscale(v, a.step_sq), a.ids[gr]) : kNoKey;
-          uint64_t *Eq = E + (16 * w + sq) * SC;
-          TopList<SC> tl;
-#pragma unroll
-          for (int r = 0; r < SC / 64; ++r) tl.e[r] = Eq[lane + 64 * r];
-          if (__popcll(m) <= 4) {
-            uint64_t mm = m;
-            while (mm) {
-              const int b = __ffsll(static_cast<long long>(mm)) - 1;
-              mm &= mm - 1;
-              top_insert<SC>(tl, __shfl(key, b), a.S);
-            }
-          } else {
-            top_merge<SC>(tl, key, a.S);
-          }
-#pragma unroll
-          for (int r = 0; r < SC / 64; ++r) Eq[lane + 64 * r] = tl.e[r];
-        }
-      }
-    }
-    // flush each query's list into its pool
-    for (int sq = 0; sq < wq; ++sq) {
-      const int slot = 16 * w + sq;
-      const uint32_t qid = s_qid[slot];
-      const uint64_t *Eq = E + slot * SC;
-      const uint64_t g = __atomic_load_n(&a.thr[qid], __ATOMIC_RELAXED);
-      uint32_t m = 0;
-      uint64_t ev[SC / 64];
-#pragma unroll
-      for (int r = 0; r < SC / 64; ++r) {
-        ev[r] = Eq[lane + 64 * r];
-        m += __popcll(ballot(ev[r] != kNoKey && ev[r] <= g));
-      }
-      if (m == 0) continue;
-      uint32_t off = 0;
-      if (lane == 0) off = atomicAdd(&a.pool_cnt[qid], m);
-      off = __shfl(off, 0);
-      const uint64_t base = a.pool_off[qid], cap = a.pool_off[qid + 1] - base;
-      if (off + m > cap) {
-        if (lane == 0) atomicOr(&a.state[1], 1u);
-        continue;
-      }
-#pragma unroll
-      for (int r = 0; r < SC / 64; ++r) {
-        const uint32_t i = lane + 64 * r;
-        if (i < m) a.pool[base + off + i] = ev[r];
-      }
-      const uint64_t ks = Eq[a.S - 1];
-      if (lane == 0 && ks != kNoKey) atomicMin(reinterpret_cast<unsigned long long *>(&a.thr[qid]),
-                                               static_cast<unsigned long long>(ks));
-    }
-  }
-}
-
-// --------------------------------------------------------------------------
-// Final survivors, refinement and top-k (src/index.cpp:1046-1127). One wave
-// per query:
-//  1. the S smallest keys of the query's pool (exactly the S smallest ranking
-//     keys of all its candidates);
-//  2. exact distances of the survivors: rows gathered by id, the four byte
-//     products on MFMA (A = 16 survivor rows of one plane, B = the query's high
-//     and low bytes as columns 0 and 1), combined in int64;
-//  3. the reference's arrival rule: survivors in descending (rank, id) order
-//     enter a k-entry max-heap of (distance, id) while it has room, then
-//     replace its maximum iff their distance is strictly smaller;
-//  4. output ascending by (distance, id), padded with FLT_MAX / UINT32_MAX
-//     (src/index.cpp:384-398).
-struct RefineArgs {
-  const uint64_t *pool;
-  const uint64_t *pool_off;
-  const uint32_t *pool_cnt;
-  const uint64_t *thr;
-  uint32_t B, S, K;
-  const int8_t *hi, *lo;     // N x dc
-  const int64_t *norm;       // N
-  const uint32_t *id_row;    // npts
-  uint32_t npts, dc;
-  const int8_t *qhi, *qlo;   // B x dc
-  const int64_t *qnorm;      // B
-  double step_sq;
-  uint32_t *out_ids;
-  float *out_dists;
-  uint64_t *surv;            // B x S or null
-  uint32_t *nsurv;           // B or null
-};
-
-template <int SC>
-__device__ inline uint64_t wave_max_list(const TopList<SC> &t) {
-  uint64_t m = 0;
-#pragma unroll
-  for (int r = 0; r < SC / 64; ++r) m = umax64(m, t.e[r] == kNoKey ? 0ull : t.e[r]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = umax64(m, __shfl_xor(m, o));
-  return m;
-}
-
-template <int SC>
-__global__ __launch_bounds__(256) void k_refine(RefineArgs a) {
-  __shared__ uint64_t ex[4][SC];
-  const int lane = lane_id(), w = threadIdx.x >> 6;
-  const uint32_t q = blockIdx.x * 4 + w;
-  if (q >= a.B) return;
-  // 1. survivors
-  TopList<SC> sv;
-#pragma unroll
-  for (int r = 0; r < SC / 64; ++r) sv.e[r] = kNoKey;
-  {
-    const uint64_t base = a.pool_off[q];
-    const uint64_t cap = a.pool_off[q + 1] - base;
-    const uint32_t m = static_cast<uint32_t>(min(static_cast<uint64_t>(a.pool_cnt[q]), cap));
-    const uint64_t tq = a.thr[q];
-    for (uint32_t b = 0; b < m; b += 64) {
-      uint64_t k = b + lane < m ? a.pool[base + b + lane] : kNoKey;
-      if (k > tq) k = kNoKey;
-      const uint64_t mk = ballot(k != kNoKey);
-      if (!mk) continue;
-      if (__popcll(mk) <= 4) {
-        uint64_t mm = mk;
-        while (mm) {
-          const int bb = __ffsll(static_cast<long long>(mm)) - 1;
-          mm &= mm - 1;
-          top_insert<SC>(sv, __shfl(k, bb), a.S);
-        }
-      } else {
-        top_merge<SC>(sv, k, a.S);
-      }
-    }
-  }
-  const uint32_t ns = top_count<SC>(sv);
-  if (a.surv) {
-#pragma unroll
-    for (int r = 0; r < SC / 64; ++r) {
-      const uint32_t i = lane + 64 * r;
-      if (i < a.S) a.surv[static_cast<size_t>(q) * a.S + i] = sv.e[r];
-    }
-  }
-  if (a.nsurv && lane == 0) a.nsurv[q] = ns;
-
-  // 2. exact distances
-  const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
-  const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
-  const int64_t qn = a.qnorm[q];
-  const int koff = 16 * (lane >> 4), col = lane & 15;
-  for (uint32_t t0 = 0; t0 < ns; t0 += 16) {
-    const uint32_t s = t0 + (lane & 15);
-    const uint64_t key = top_get<SC>(sv, min(s, static_cast<uint32_t>(SC - 1)));
-    const uint32_t id = static_cast<uint32_t>(key);
-    const bool sok = s < ns && id < a.npts;
-    const uint32_t row = sok ? a.id_row[id] : 0xffffffffu;
-    const bool rok = sok && row != 0xffffffffu;
-    v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
-    for (uint32_t kc = 0; kc < a.dc; kc += 64) {
-      const v4i xh = rok ? load16(a.hi + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-      const v4i xl = rok ? load16(a.lo + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-      const v4i b = col == 0 ? load16(qh + kc + koff) : (col == 1 ? load16(ql + kc + koff) : v4i{0, 0, 0, 0});
-      ah = mfma(xh, b, ah);
-      al = mfma(xl, b, al);
-    }
-    // survivor i = lane (< 16): C[i][0] in lane 16 (i/4), element i % 4; C[i][1] one lane up
-    const int i = lane & 15;
-    const int src = 16 * (i >> 2);
-    int32_t hh = 0, hl = 0, lh = 0, ll = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int32_t a0 = __shfl(ah[r], src), a1 = __shfl(ah[r], src + 1);
-      const int32_t b0 = __shfl(al[r], src), b1 = __shfl(al[r], src + 1);
-      if ((i & 3) == r) {
-        hh = a0;
-        hl = a1;
-        lh = b0;
-        ll = b1;
-      }
-    }
-    const uint32_t si = t0 + i;
-    const bool mine_ok = __shfl(static_cast<int>(rok), i) != 0;
-    const uint32_t myrow = __shfl(row, i);
-    const uint32_t myid = __shfl(id, i);
-    if (lane < 16 && si < ns) {
-      uint64_t e = kNoKey;
-      if (mine_ok) {
-        const int64_t dot = static_cast<int64_t>(hh) * 65536 + (static_cast<int64_t>(hl) + lh) * 256 +
-                            static_cast<int64_t>(ll);
-        e = make_key(fscale(qn + a.norm[myrow] - 2 * dot, a.step_sq), myid);
-      }
-      ex[w][si] = e;
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-
-  // 3. arrival rule over survivors in descending ranking-key order
-  TopList<SC> h;
-#pragma unroll
-  for (int r = 0; r < SC / 64; ++r) h.e[r] = kNoKey;
-  uint32_t nh = 0;
-  uint64_t mx = 0;
-  for (int t = static_cast<int>(ns) - 1; t >= 0; --t) {
-    const uint64_t c = ex[w][t];
-    if (c == kNoKey) continue;
-    if (nh < a.K) {
-      if (lane == static_cast<int>(nh % 64)) h.e[nh / 64] = c;
-      ++nh;
-      if (nh == a.K) mx = wave_max_list<SC>(h);
-    } else if (key_float(c) < key_float(mx)) {
-      bool done = false;
-#pragma unroll
-      for (int r = 0; r < SC / 64; ++r) {
-        const uint64_t hit = ballot(h.e[r] == mx);
-        if (!done && hit) {
-          const int hl2 = __ffsll(static_cast<long long>(hit)) - 1;
-          if (lane == hl2) h.e[r] = c;
-          done = true;
-        }
-      }
-      mx = wave_max_list<SC>(h);
-    }
-  }
-  // 4. ascending (distance, id)
-  if (SC == 64) {
-    h.e[0] = wave_sort64(h.e[0]);
-  } else {
-    const uint64_t s0 = wave_sort64(h.e[0]);
-    const uint64_t s1 = __shfl(wave_sort64(h.e[SC / 64 - 1]), 63 - lane);
-    h.e[0] = wave_bitonic_merge64(umin64(s0, s1));
-    h.e[SC / 64 - 1] = wave_bitonic_merge64(umax64(s0, s1));
-  }
-#pragma unroll
-  for (int r = 0; r < SC / 64; ++r) {
-    const uint32_t g = lane + 64 * r;
-    if (g < a.K) {
-      const bool ok = g < nh;
-      a.out_ids[static_cast<size_t>(q) * a.K + g] = ok ? static_cast<uint32_t>(h.e[r]) : 0xffffffffu;
-      a.out_dists[static_cast<size_t>(q) * a.K + g] = ok ? key_float(h.e[r]) : FLT_MAX;
-    }
-  }
+// Per-batch resets of each query's pool and threshold.
+__global__ void k_query_init(uint32_t B, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= B) return;
+  pool_cnt[q] = 0;
+  pool_sorted[q] = 0;
+  thr[q] = kNoKey;
 }
 
 // --------------------------------------------------------------------------
@@ -1007,78 +594,25 @@ void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, u
   hipLaunchKernelGGL(k_select, dim3(B), dim3(256), sm, s, dist, B, nlist, np, probes, probe_d);
 }
 
+uint32_t plan_slots(uint32_t nlist, uint32_t np) {
+  const uint32_t nb = 32u - static_cast<uint32_t>(__builtin_clz(np));  // bands of ranks 0 .. np - 1
+  return nb * nlist;
+}
+
 void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t S, uint32_t *gcnt, uint32_t *qoff,
-                 uint32_t *item_off, uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint64_t *pool_off,
-                 uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, hipStream_t s) {
+                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
+                 uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
+                 uint64_t *thr, uint32_t *state, uint32_t nstate, hipStream_t s) {
   const uint64_t t = static_cast<uint64_t>(B) * np;
   const uint32_t g = static_cast<uint32_t>((t + 255) / 256);
-  hipLaunchKernelGGL(k_fill32, dim3((nlist + 255) / 256), dim3(256), 0, s, gcnt, nlist, 0u);
-  hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, probes, npq, B, np, gcnt);
-  hipLaunchKernelGGL(k_plan_lists, dim3(1), dim3(1024), 0, s, gcnt, list_off, nlist, qc, rt, qoff, item_off, gfill,
-                     totals);
-  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, probes, npq, B, np, qoff, gfill, qlist);
-  hipLaunchKernelGGL(k_plan_queries, dim3(1), dim3(1024), 0, s, probes, npq, B, np, list_off, rt, S, pool_off,
-                     pool_cnt, thr, state);
-}
-
-size_t rank_smem(uint32_t P, uint32_t S) {
-  const uint32_t KC = P / 64, SC = S <= 64 ? 64 : 128;
-  return static_cast<size_t>(4) * KC * 1024 + 4 * 16 * 64 * 4 + static_cast<size_t>(RK_QC) * SC * 8;
-}
-
-template <int KC, int SC>
-static void rank_launch(const RankArgs &a, uint32_t grid, hipStream_t s) {
-  const size_t sm = rank_smem(KC * 64, SC);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rank<KC, SC>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(sm));
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_rank<KC, SC>), dim3(grid), dim3(256), sm, s, a);
-}
-
-template <int SC>
-static bool rank_dispatch_kc(const RankArgs &a, uint32_t grid, hipStream_t s) {
-  switch (a.P / 64) {
-#define KC_CASE(n) \
-  case n:          \
-    rank_launch<n, SC>(a, grid, s); \
-    return true;
-    KC_CASE(1) KC_CASE(2) KC_CASE(3) KC_CASE(4) KC_CASE(5) KC_CASE(6) KC_CASE(7) KC_CASE(8)
-    KC_CASE(9) KC_CASE(10) KC_CASE(11) KC_CASE(12) KC_CASE(13) KC_CASE(14) KC_CASE(15) KC_CASE(16)
-#undef KC_CASE
-    default:
-      return false;
-  }
-}
-
-bool launch_rank(const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
-                 const uint32_t *gcnt, const uint32_t *qoff, const uint32_t *qlist, const uint32_t *item_off,
-                 const uint32_t *totals, uint32_t rt, uint32_t S, double step_sq, uint64_t *pool,
-                 const uint64_t *pool_off, uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, uint32_t grid,
-                 hipStream_t s) {
-  RankArgs a{plane, xnorm, ids, list_off, nlist, P, qhi, dc, qnorm, gcnt, qoff, qlist, item_off, totals, rt, S,
-             static_cast<int32_t>(P * 65025u), step_sq, pool, pool_off, pool_cnt, thr, state};
-  if (S <= 64) return rank_dispatch_kc<64>(a, grid, s);
-  if (S <= 128) return rank_dispatch_kc<128>(a, grid, s);
-  return false;
-}
-
-bool launch_refine(const uint64_t *pool, const uint64_t *pool_off, const uint32_t *pool_cnt, const uint64_t *thr,
-                   uint32_t B, uint32_t S, uint32_t K, const int8_t *hi, const int8_t *lo, const int64_t *norm,
-                   const uint32_t *id_row, uint32_t npts, uint32_t dc, const int8_t *qhi, const int8_t *qlo,
-                   const int64_t *qnorm, double step_sq, uint32_t *out_ids, float *out_dists, uint64_t *surv,
-                   uint32_t *nsurv, hipStream_t s) {
-  RefineArgs a{pool, pool_off, pool_cnt, thr, B, S, K, hi, lo, norm, id_row, npts, dc, qhi, qlo, qnorm, step_sq,
-               out_ids, out_dists, surv, nsurv};
-  const dim3 grid((B + 3) / 4);
-  if (S <= 64) hipLaunchKernelGGL(k_refine<64>, grid, dim3(256), 0, s, a);
-  else if (S <= 128) hipLaunchKernelGGL(k_refine<128>, grid, dim3(256), 0, s, a);
-  else return false;
-  return true;
+  const uint32_t nslots = plan_slots(nlist, np);
+  hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, gcnt, nslots, 0u);
+  hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
+  hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, gcnt);
+  hipLaunchKernelGGL(k_plan_lists, dim3(1), dim3(1024), 0, s, gcnt, list_off, nlist, nslots, qc, rt, qoff, item_off,
+                     gfill, totals);
+  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, qoff, gfill, qlist);
+  hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr);
 }
 
 void launch_sq16_pairs(const int8_t *qh, const int8_t *ql, const int64_t *qn, const int8_t *xh, const int8_t *xl,

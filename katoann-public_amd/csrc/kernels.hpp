@@ -19,18 +19,30 @@ void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, u
 size_t select_smem(uint32_t nlist, uint32_t np);
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
                    hipStream_t s);
+// (rank band, list) slots of the work plan for nprobe np: band b holds probe
+// ranks [2^b - 1, 2^(b+1) - 1).
+uint32_t plan_slots(uint32_t nlist, uint32_t np);
+inline uint32_t plan_bands(uint32_t np) { return 32u - static_cast<uint32_t>(__builtin_clz(np)); }
 void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t S, uint32_t *gcnt, uint32_t *qoff,
-                 uint32_t *item_off, uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint64_t *pool_off,
-                 uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, hipStream_t s);
-size_t rank_smem(uint32_t P, uint32_t S);
-bool launch_rank(const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
-                 const uint32_t *gcnt, const uint32_t *qoff, const uint32_t *qlist, const uint32_t *item_off,
-                 const uint32_t *totals, uint32_t rt, uint32_t S, double step_sq, uint64_t *pool,
-                 const uint64_t *pool_off, uint32_t *pool_cnt, uint64_t *thr, uint32_t *state, uint32_t grid,
+                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
+                 uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
+                 uint64_t *thr, uint32_t *state, uint32_t nstate, hipStream_t s);
+void launch_seed(const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, const uint64_t *list_off,
+                 uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const uint32_t *probes,
+                 const uint32_t *npq, uint32_t np, uint32_t B, uint32_t S, double step_sq, uint64_t *thr,
                  hipStream_t s);
-bool launch_refine(const uint64_t *pool, const uint64_t *pool_off, const uint32_t *pool_cnt, const uint64_t *thr,
+size_t band_smem(uint32_t P);
+void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t rt, uint32_t cap, uint4 *items,
+                  hipStream_t s);
+bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, uint32_t nlist,
+                      uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const uint32_t *qlist,
+                      const uint32_t *item_off, const uint4 *items, uint32_t stride, double step_sq, uint64_t *pool,
+                      uint32_t *pool_cnt, const uint64_t *thr, uint32_t *state, uint32_t grid, hipStream_t s);
+uint32_t refine_cap(uint32_t S);
+bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, uint32_t B, uint32_t S,
+                   uint32_t stride, hipStream_t s);
+bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, const uint64_t *thr,
                    uint32_t B, uint32_t S, uint32_t K, const int8_t *hi, const int8_t *lo, const int64_t *norm,
                    const uint32_t *id_row, uint32_t npts, uint32_t dc, const int8_t *qhi, const int8_t *qlo,
                    const int64_t *qnorm, double step_sq, uint32_t *out_ids, float *out_dists, uint64_t *surv,

@@ -300,7 +300,7 @@ class Index:
     def last_work(self):
         out = np.zeros(4, np.uint64)
         _check(lib().katoann_last_work(self._h, _ptr(out), 4))
-        return dict(pairs=int(out[0]), items=int(out[1]))
+        return dict(pairs=int(out[0]), items=int(out[1]), kept=int(out[2]), bands=int(out[3]))
 
     def set_rank_rows(self, rows: int):
         _check(lib().katoann_set_rank_rows(self._h, rows))

@@ -80,7 +80,8 @@ def check_against_oracle(ix, ox, q, k, nprobe, S, ranking="prefix", npq=None):
 
 
 @pytest.mark.parametrize("k,nprobe,S", [(10, 16, 32), (10, 1, 10), (10, 64, 32), (100, 16, 100),
-                                        (5, 8, 128), (32, 256, 64)])
+                                        (5, 8, 128), (32, 256, 64), (10, 64, 256), (10, 128, 500),
+                                        (10, 32, 1000), (200, 16, 200)])
 def test_mixture_vs_oracle(mixture_index, k, nprobe, S):
     prefix, base, q = mixture_index
     print("mixture", k, nprobe, S, flush=True)
