@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--kmeans-iters", type=int, default=10)
     ap.add_argument("--rank-rows", type=int, default=0)
+    ap.add_argument("--band-shift", type=int, default=0, help="probe-rank bands grow by 2**shift (default: library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1024)
     args = ap.parse_args()
@@ -96,6 +97,8 @@ def main():
         time.time() - t0, sizes.min(), int(np.median(sizes)), sizes.max()))
     if args.rank_rows:
         ix.set_rank_rows(args.rank_rows)
+    if args.band_shift:
+        ix.set_band_factor(args.band_shift)
     t0 = time.time()
     gt = datasets.ground_truth_torch(base, queries, 10)
     log("ground truth %.1fs" % (time.time() - t0))

@@ -30,7 +30,6 @@ namespace {
 constexpr uint32_t kMaxDim = 4096;          // kernels.h:24
 constexpr uint32_t kDimcMagic = 0x434D4944u;  // index.cpp:27
 constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
-constexpr uint32_t kQueriesPerItem = 64;    // ranking kernel's query chunk
 constexpr uint32_t kMaxSurvivors = 1024;
 constexpr uint32_t kPoolAllowance = 1024;  // pool slots per query beyond S for one band's kept keys
 constexpr uint32_t kMaxProbes = 4096;
@@ -83,7 +82,7 @@ struct SearchCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[KATOANN_NSTAGES + 1] = {};
   DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, gfill, item_off, qlist,
-      totals, pool_cnt, pool_sorted, thr, state, pool, ids, dists, surv, nsurv, codes, items;
+      totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items;
   std::vector<uint32_t> host_totals;
   SearchCtx() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -108,6 +107,7 @@ struct katoann_index {
   std::vector<uint64_t> list_off;
   uint64_t nrows = 0;
   uint32_t rank_rows = 1024;
+  uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
   std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
   uint32_t tile_prefix_rows = 0;
 
@@ -435,26 +435,30 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.dist.ensure(static_cast<size_t>(B) * nlist * 4);
   c.probes.ensure(static_cast<size_t>(B) * np * 4);
   c.probe_d.ensure(static_cast<size_t>(B) * np * 4);
-  const uint32_t nslots = katoann_hip::plan_slots(nlist, np);
+  const uint32_t shift = ix.band_shift;
+  const uint32_t qc = katoann_hip::rank_queries_per_item(P);
+  if (qc == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+  const uint32_t nslots = katoann_hip::plan_slots(nlist, np, shift);
   c.gcnt.ensure(nslots * 4ull);
   c.qoff.ensure(nslots * 4ull);
   c.gfill.ensure(nslots * 4ull);
   c.item_off.ensure((nslots + 1ull) * 4);
   c.qlist.ensure(static_cast<size_t>(B) * np * 4);
   c.totals.ensure(16);
-  c.pool_cnt.ensure(B * 4ull);
+  c.pool_cnt.ensure(B * 4ull * katoann_hip::kCntStride);
   c.pool_sorted.ensure(B * 4ull);
   c.thr.ensure(B * 8ull);
-  const uint32_t nbands = katoann_hip::plan_bands(np);
+  c.lbound.ensure(B * 4ull);
+  const uint32_t nbands = katoann_hip::plan_bands(np, shift);
   const uint32_t nstate = 3 + nbands;
   c.state.ensure(nstate * 4ull);
   const uint32_t rt = ix.rank_rows;
   const uint64_t max_rows = mix.max_rows_per_query(np);
-  // items <= sum over lists of row tiles x (ceil(B / 64) + bands): a list's query
+  // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
   // groups over all bands hold at most B queries
   uint64_t tiles_total = 0;
   for (uint32_t cc = 0; cc < nlist; ++cc) tiles_total += (ix.sizes[cc] + rt - 1) / rt;
-  const uint64_t item_cap64 = tiles_total * ((B + kQueriesPerItem - 1) / kQueriesPerItem + nbands);
+  const uint64_t item_cap64 = tiles_total * ((B + qc - 1) / qc + nbands);
   if (item_cap64 > 0x7fffffffull) throw std::invalid_argument("batch too large for the ranking plan");
   const uint32_t item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
   c.items.ensure(static_cast<size_t>(item_cap) * 16);
@@ -473,7 +477,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   auto mark = [&](int i) {
     if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
   };
-  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 4;
+  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 2;  // ranking blocks resident per CU
   uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
   float *ddst = dev_io ? out_dists : c.dists.as<float>();
   const float *yq = c.q.as<float>();
@@ -504,29 +508,27 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
                                s);
     mark(3);
-    katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist, kQueriesPerItem,
-                             rt, c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
+    katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist, qc, rt,
+                             c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
                              c.gfill.as<uint32_t>(), c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
                              c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(), c.thr.as<uint64_t>(),
-                             c.state.as<uint32_t>(), nstate, s);
+                             c.lbound.as<int32_t>(), c.state.as<uint32_t>(), nstate, shift, s);
     katoann_hip::launch_items(c.item_off.as<uint32_t>(), nslots, nlist, ix.list_off_d.as<uint64_t>(),
-                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), rt, item_cap, c.items.as<uint4>(), s);
+                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), qc, rt, item_cap, c.items.as<uint4>(), s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
     const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
-    katoann_hip::launch_seed(plane, xnorm, ix.ids.as<uint32_t>(), ix.list_off_d.as<uint64_t>(), P,
-                             c.qhi.as<int8_t>(), dc, qn, c.probes.as<uint32_t>(), npq, np, B, S, ix.step_sq,
-                             c.thr.as<uint64_t>(), s);
     for (uint32_t band = 0; band < nbands; ++band) {
       if (!katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, c.qhi.as<int8_t>(),
-                                         dc, qn, c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                                         c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
-                                         c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(), c.state.as<uint32_t>(), grid,
-                                         s))
+                                         dc, qn, c.lbound.as<int32_t>(), c.qlist.as<uint32_t>(),
+                                         c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride, ix.step_sq,
+                                         c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(),
+                                         grid, s))
         throw std::invalid_argument("unsupported ranking shape (prefix dims)");
-      katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
-                                 c.thr.as<uint64_t>(), B, S, stride, s);
+      if (!katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
+                                      c.thr.as<uint64_t>(), c.lbound.as<int32_t>(), B, S, stride, P, ix.step_sq, s))
+        throw std::invalid_argument("unsupported survivor count");
     }
     mark(5);
     if (!katoann_hip::launch_refine(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
@@ -543,7 +545,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     HIPCHECK(hipStreamSynchronize(s));
     return st[0] == 0;
   };
-  if (!run(kPoolAllowance) && !run(max_rows)) {
+  // the first bands run without a threshold (or with one from few candidates):
+  // room for every row of a query's 1 + F nearest lists
+  const uint64_t allowance =
+      std::min(max_rows, std::max<uint64_t>(kPoolAllowance, mix.max_rows_per_query(std::min(np, 1u + (1u << shift)))));
+  if (!run(allowance) && (allowance >= max_rows || !run(max_rows))) {
     ix.release(std::move(ctx));
     throw std::runtime_error("survivor pool overflow in the ranking pass");
   }
@@ -778,6 +784,14 @@ int katoann_set_rank_rows(katoann_index *h, uint32_t rows) {
   if (rows == 0 || rows % 64) throw std::invalid_argument("rank rows must be a positive multiple of 64");
   std::lock_guard<std::mutex> g(h->mu);
   h->rank_rows = rows;
+  KATOANN_CATCH
+}
+
+int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
+  KATOANN_TRY
+  if (log2_factor < 1 || log2_factor > 4) throw std::invalid_argument("band factor must be 2, 4, 8 or 16");
+  std::lock_guard<std::mutex> g(h->mu);
+  h->band_shift = log2_factor;
   KATOANN_CATCH
 }
 

@@ -399,15 +399,19 @@ __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
 // Work is grouped by (probe-rank band, list): band b holds ranks [2^b - 1, 2^(b+1) - 1),
 // so a query's nearest lists come first in the queue and its threshold is
 // tight when its farther lists are ranked.
-__device__ __forceinline__ uint32_t band_of(uint32_t p) { return 31u - __clz(p + 1u); }
+// Probe-rank band of rank p for band factor F = 2^shift: band b holds ranks
+// [(F^b - 1) / (F - 1), (F^(b+1) - 1) / (F - 1)), i.e. b = floor(log_F((F - 1) p + 1)).
+__device__ __forceinline__ uint32_t band_of(uint32_t p, uint32_t shift) {
+  return (31u - __clz(((1u << shift) - 1u) * p + 1u)) / shift;
+}
 
 __global__ void k_hist(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                       uint32_t np, uint32_t nlist, uint32_t *gcnt) {
+                       uint32_t np, uint32_t nlist, uint32_t shift, uint32_t *gcnt) {
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
   if (npq && p >= npq[q]) return;
-  atomicAdd(&gcnt[band_of(p) * nlist + probes[t]], 1u);
+  atomicAdd(&gcnt[band_of(p, shift) * nlist + probes[t]], 1u);
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -471,23 +475,26 @@ __global__ __launch_bounds__(1024) void k_plan_lists(const uint32_t *__restrict_
 }
 
 __global__ void k_scatter(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                          uint32_t np, uint32_t nlist, const uint32_t *__restrict__ qoff, uint32_t *gfill,
-                          uint32_t *qlist) {
+                          uint32_t np, uint32_t nlist, uint32_t shift, const uint32_t *__restrict__ qoff,
+                          uint32_t *gfill, uint32_t *qlist) {
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
   if (npq && p >= npq[q]) return;
-  const uint32_t c = band_of(p) * nlist + probes[t];
+  const uint32_t c = band_of(p, shift) * nlist + probes[t];
   qlist[qoff[c] + atomicAdd(&gfill[c], 1u)] = q;
 }
 
-// Per-batch resets of each query's pool and threshold.
-__global__ void k_query_init(uint32_t B, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr) {
+// Per-batch resets of each query's pool and threshold (none yet: every
+// candidate of the first band is kept).
+__global__ void k_query_init(uint32_t B, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr,
+                             int32_t *lbound) {
   const uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= B) return;
-  pool_cnt[q] = 0;
+  pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride] = 0;
   pool_sorted[q] = 0;
   thr[q] = kNoKey;
+  lbound[q] = 0x7fffffff;
 }
 
 // --------------------------------------------------------------------------
@@ -594,25 +601,24 @@ void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, u
   hipLaunchKernelGGL(k_select, dim3(B), dim3(256), sm, s, dist, B, nlist, np, probes, probe_d);
 }
 
-uint32_t plan_slots(uint32_t nlist, uint32_t np) {
-  const uint32_t nb = 32u - static_cast<uint32_t>(__builtin_clz(np));  // bands of ranks 0 .. np - 1
-  return nb * nlist;
-}
+uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift) { return plan_bands(np, shift) * nlist; }
 
 void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
                  uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
                  uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
-                 uint64_t *thr, uint32_t *state, uint32_t nstate, hipStream_t s) {
+                 uint64_t *thr, int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, hipStream_t s) {
   const uint64_t t = static_cast<uint64_t>(B) * np;
   const uint32_t g = static_cast<uint32_t>((t + 255) / 256);
-  const uint32_t nslots = plan_slots(nlist, np);
+  const uint32_t nslots = plan_slots(nlist, np, shift);
   hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, gcnt, nslots, 0u);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
-  hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, gcnt);
+  hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, shift, gcnt);
   hipLaunchKernelGGL(k_plan_lists, dim3(1), dim3(1024), 0, s, gcnt, list_off, nlist, nslots, qc, rt, qoff, item_off,
                      gfill, totals);
-  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, qoff, gfill, qlist);
-  hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr);
+  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, shift, qoff, gfill,
+                     qlist);
+  hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr,
+                     lbound);
 }
 
 void launch_sq16_pairs(const int8_t *qh, const int8_t *ql, const int64_t *qn, const int8_t *xh, const int8_t *xl,

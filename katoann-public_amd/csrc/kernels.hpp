@@ -7,6 +7,10 @@
 
 namespace katoann_hip {
 
+// Each query's pool counter sits alone in a 128-byte line (the ranking pass
+// updates it with atomics from every CU).
+constexpr uint32_t kCntStride = 32;
+
 void launch_normalize(float *x, uint32_t n, uint32_t d, hipStream_t s);
 void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float *Y, hipStream_t s);
 void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32_t P, float lo,
@@ -19,29 +23,31 @@ void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, u
 size_t select_smem(uint32_t nlist, uint32_t np);
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
                    hipStream_t s);
-// (rank band, list) slots of the work plan for nprobe np: band b holds probe
-// ranks [2^b - 1, 2^(b+1) - 1).
-uint32_t plan_slots(uint32_t nlist, uint32_t np);
-inline uint32_t plan_bands(uint32_t np) { return 32u - static_cast<uint32_t>(__builtin_clz(np)); }
+// Probe-rank bands for band factor F = 2^shift: band b holds ranks
+// [(F^b - 1) / (F - 1), (F^(b+1) - 1) / (F - 1)); the work plan has one
+// (band, list) slot per band and list.
+inline uint32_t plan_bands(uint32_t np, uint32_t shift) {
+  const uint32_t v = ((1u << shift) - 1u) * (np - 1u) + 1u;
+  return (31u - static_cast<uint32_t>(__builtin_clz(v))) / shift + 1u;
+}
+uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift);
 void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
                  uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
                  uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
-                 uint64_t *thr, uint32_t *state, uint32_t nstate, hipStream_t s);
-void launch_seed(const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, const uint64_t *list_off,
-                 uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const uint32_t *probes,
-                 const uint32_t *npq, uint32_t np, uint32_t B, uint32_t S, double step_sq, uint64_t *thr,
-                 hipStream_t s);
-size_t band_smem(uint32_t P);
+                 uint64_t *thr, int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, hipStream_t s);
+// Queries per ranking item for ranking width P (0 if P is unsupported).
+uint32_t rank_queries_per_item(uint32_t P);
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
-                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t rt, uint32_t cap, uint4 *items,
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint4 *items,
                   hipStream_t s);
 bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, uint32_t nlist,
-                      uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const uint32_t *qlist,
-                      const uint32_t *item_off, const uint4 *items, uint32_t stride, double step_sq, uint64_t *pool,
-                      uint32_t *pool_cnt, const uint64_t *thr, uint32_t *state, uint32_t grid, hipStream_t s);
+                      uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const int32_t *lbound,
+                      const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,
+                      double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state, uint32_t grid,
+                      hipStream_t s);
 uint32_t refine_cap(uint32_t S);
-bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, uint32_t B, uint32_t S,
-                   uint32_t stride, hipStream_t s);
+bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
+                   uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s);
 bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, const uint64_t *thr,
                    uint32_t B, uint32_t S, uint32_t K, const int8_t *hi, const int8_t *lo, const int64_t *norm,
                    const uint32_t *id_row, uint32_t npts, uint32_t dc, const int8_t *qhi, const int8_t *qlo,

@@ -83,6 +83,7 @@ def lib():
         L.katoann_stage_times.argtypes = [P, P, i32]
         L.katoann_last_work.argtypes = [P, P, i32]
         L.katoann_set_rank_rows.argtypes = [P, u32]
+        L.katoann_set_band_factor.argtypes = [P, u32]
         L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
         L.katoann_sq16_high_l2.argtypes = [P, P, P, P, f64, u32, u32, u32, P, u32]
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
@@ -304,6 +305,10 @@ class Index:
 
     def set_rank_rows(self, rows: int):
         _check(lib().katoann_set_rank_rows(self._h, rows))
+
+    def set_band_factor(self, log2_factor: int):
+        """Probe-rank bands of the ranking pass grow by 2**log2_factor (1..4)."""
+        _check(lib().katoann_set_band_factor(self._h, log2_factor))
 
 
 # ---- kernel-level entry points (reference include/katoann/kernels.h:34-52)
