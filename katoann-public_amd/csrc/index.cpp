@@ -81,7 +81,7 @@ void upload(DevBuf &b, const T *src, size_t count) {
 struct SearchCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[KATOANN_NSTAGES + 1] = {};
-  DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, gfill, item_off, qlist,
+  DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
       totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items;
   std::vector<uint32_t> host_totals;
   SearchCtx() {
@@ -441,7 +441,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   const uint32_t nslots = katoann_hip::plan_slots(nlist, np, shift);
   c.gcnt.ensure(nslots * 4ull);
   c.qoff.ensure(nslots * 4ull);
-  c.gfill.ensure(nslots * 4ull);
+  c.pscratch.ensure(katoann_hip::plan_scratch_words(B, nlist, np, shift) * 4);
   c.item_off.ensure((nslots + 1ull) * 4);
   c.qlist.ensure(static_cast<size_t>(B) * np * 4);
   c.totals.ensure(16);
@@ -508,9 +508,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
                                s);
     mark(3);
-    katoann_hip::launch_plan(c.probes.as<uint32_t>(), npq, B, np, ix.list_off_d.as<uint64_t>(), nlist, qc, rt,
+    katoann_hip::launch_plan(c.dist.as<float>(), c.probes.as<uint32_t>(), c.probe_d.as<float>(), npq, B, np,
+                             ix.list_off_d.as<uint64_t>(), nlist, qc, rt, c.pscratch.as<uint32_t>(),
                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                             c.gfill.as<uint32_t>(), c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
+                             c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
                              c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(), c.thr.as<uint64_t>(),
                              c.lbound.as<int32_t>(), c.state.as<uint32_t>(), nstate, shift, s);
     katoann_hip::launch_items(c.item_off.as<uint32_t>(), nslots, nlist, ix.list_off_d.as<uint64_t>(),

@@ -283,56 +283,105 @@ __global__ __launch_bounds__(256) void k_coarse(const int8_t *__restrict__ qhi, 
 // --------------------------------------------------------------------------
 // The np smallest (float, list) keys of each query's coarse row, ascending
 // (canonical form of src/index.cpp:862-874). One 256-thread block per query:
-// 4-pass radix select on the float bits, then the ties at the pivot taken in
-// list order, then a bitonic sort of the np keys in LDS.
+//  1. radix select of the np-th smallest value on the float bits relative to
+//     the row minimum (non-negative floats order like their bits), 11-bit
+//     digits from the highest set bit of the row's range: at most 3 passes,
+//     and the histogram bins spread over the row's own value range;
+//  2. the values below the pivot, then the ties at the pivot in list order;
+//  3. a bitonic sort of the np keys in LDS.
 constexpr uint32_t SEL_CACHE = 16384;  // coarse row cached in LDS up to this nlist
 constexpr uint32_t SEL_MAXNP = 4096;
+constexpr int SEL_BINS = 2048;
+
+// Block-wide exclusive scan of 8 consecutive bins per thread (256 threads,
+// 2048 bins): finds the bin where the running count reaches rank.
+__device__ inline void sel_find(const uint32_t *hist, uint32_t rank, uint32_t *wsum, uint32_t *sh_digit,
+                                uint32_t *sh_rank) {
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  uint32_t h[8], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    h[i] = hist[8 * tid + i];
+    sum += h[i];
+  }
+  uint32_t inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t cum = inc - sum;
+  for (int ww = 0; ww < w; ++ww) cum += wsum[ww];
+  if (cum < rank && rank <= cum + sum) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (cum < rank && rank <= cum + h[i]) {
+        *sh_digit = 8 * tid + i;
+        *sh_rank = rank - cum;
+      }
+      cum += h[i];
+    }
+  }
+  __syncthreads();
+}
 
 __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
                                                 uint32_t np, uint32_t *__restrict__ probes,
                                                 float *__restrict__ probe_d) {
-  extern __shared__ unsigned char smem[];
-  uint32_t *cache = reinterpret_cast<uint32_t *>(smem);
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const bool cached = nlist <= SEL_CACHE;
-  uint64_t *sel = reinterpret_cast<uint64_t *>(smem + (cached ? SEL_CACHE * 4 : 0));
-  __shared__ uint32_t hist[256];
-  __shared__ uint32_t sh_digit, sh_rank, sh_cnt, sh_taken, wsum[4];
+  uint64_t *sel = reinterpret_cast<uint64_t *>(smem);
+  uint32_t M = 1;
+  while (M < np) M <<= 1;
+  uint32_t *cache = reinterpret_cast<uint32_t *>(smem + static_cast<size_t>(M) * 8);
+  __shared__ uint32_t hist[SEL_BINS];
+  __shared__ uint32_t sh_digit, sh_rank, sh_cnt, sh_taken, wsum[4], wred[8];
   const uint32_t q = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
   const uint32_t *row = reinterpret_cast<const uint32_t *>(dist) + static_cast<size_t>(q) * nlist;
-  if (cached)
-    for (uint32_t i = tid; i < nlist; i += 256) cache[i] = row[i];
+  uint32_t vmin = 0xffffffffu, vmax = 0;
+  for (uint32_t i = tid; i < nlist; i += 256) {
+    const uint32_t v = row[i];
+    if (cached) cache[i] = v;
+    vmin = min(vmin, v);
+    vmax = max(vmax, v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), o)));
+    vmax = max(vmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmax), o)));
+  }
+  if (lane == 0) {
+    wred[w] = vmin;
+    wred[4 + w] = vmax;
+  }
   __syncthreads();
+  vmin = min(min(wred[0], wred[1]), min(wred[2], wred[3]));
+  vmax = max(max(wred[4], wred[5]), max(wred[6], wred[7]));
   auto val = [&](uint32_t i) -> uint32_t { return cached ? cache[i] : row[i]; };
 
+  // radix select on r = v - vmin
+  const uint32_t range = vmax - vmin;
+  const int nbits = range ? 32 - __clz(range) : 0;
   uint32_t prefix = 0, mask = 0, rank = np;
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[tid] = 0;
+  for (int top = nbits; top > 0; top -= 11) {
+    const int shift = max(0, top - 11);
+    const uint32_t dmask = (1u << (top - shift)) - 1u;
+    for (int i = tid; i < SEL_BINS; i += 256) hist[i] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < nlist; i += 256) {
-      const uint32_t v = val(i);
-      if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+      const uint32_t r = val(i) - vmin;
+      if ((r & mask) == prefix) atomicAdd(&hist[(r >> shift) & dmask], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
-      uint32_t cum = 0, dg = 255;
-      for (uint32_t g = 0; g < 256; ++g) {
-        if (cum + hist[g] >= rank) {
-          dg = g;
-          break;
-        }
-        cum += hist[g];
-      }
-      sh_digit = dg;
-      sh_rank = rank - cum;
-    }
-    __syncthreads();
+    sel_find(hist, rank, wsum, &sh_digit, &sh_rank);
     prefix |= sh_digit << shift;
-    mask |= 255u << shift;
+    mask |= dmask << shift;
     rank = sh_rank;
-    __syncthreads();
   }
-  const uint32_t T = prefix, need_eq = rank, n_lt = np - need_eq;
+  const uint32_t T = vmin + prefix, need_eq = rank, n_lt = np - need_eq;
   if (tid == 0) {
     sh_cnt = 0;
     sh_taken = 0;
@@ -350,7 +399,6 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
     const uint32_t i = base + tid;
     const bool f = i < nlist && val(i) == T;
     const uint64_t bal = ballot(f);
-    const int lane = lane_id(), w = tid >> 6;
     const uint32_t before = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) wsum[w] = __popcll(bal);
     __syncthreads();
@@ -361,21 +409,19 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
     if (tid == 0) sh_taken = taken + wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
   __syncthreads();
-  uint32_t M = 1;
-  while (M < np) M <<= 1;
   for (uint32_t i = np + tid; i < M; i += 256) sel[i] = kNoKey;
   __syncthreads();
+  // bitonic sort: every thread takes M / 512 compare-exchange pairs per step
   for (uint32_t k = 2; k <= M; k <<= 1)
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = tid; i < M; i += 256) {
-        const uint32_t p = i ^ j;
-        if (p > i) {
-          const uint64_t a = sel[i], b = sel[p];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            sel[i] = b;
-            sel[p] = a;
-          }
+      for (uint32_t pr = tid; pr < M / 2; pr += 256) {
+        const uint32_t i = 2 * pr - (pr & (j - 1));
+        const uint32_t p = i + j;
+        const uint64_t a = sel[i], b = sel[p];
+        const bool up = (i & k) == 0;
+        if ((a > b) == up) {
+          sel[i] = b;
+          sel[p] = a;
         }
       }
       __syncthreads();
@@ -388,30 +434,154 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
 }
 
 // --------------------------------------------------------------------------
-// Inversion (src/index.cpp:403-412) as a device histogram, scan and scatter,
-// plus the work plan: items (list, 64-query chunk, row tile), each query's
-// pool capacity (S per item it belongs to), and the per-batch state resets.
+// Inversion (src/index.cpp:403-412) and the work plan, list-major and without
+// global atomics. Work is grouped by (probe-rank band, list): for band factor
+// F = 2^shift, band b holds ranks [(F^b - 1) / (F - 1), (F^(b+1) - 1) / (F - 1)),
+// so a query's nearest lists come first in the queue and its threshold is tight
+// when its farther lists are ranked. Pair (q, c) is probed iff the key
+// (coarse distance bits, c) is at most the key of q's last probe, and its band
+// is the number of band-start keys of q at or below it; the coarse matrix is
+// read in tiles of 64 lists x 256 queries:
+//   k_inv_count  per-tile counts of every (band, list) slot (LDS),
+//   k_slot_totals per slot: the tiles' offsets within it, its query count and
+//                its items; per 1024 slots: their sums,
+//   k_slot_scan  slot and item offsets (exclusive scans),
+//   k_inv_fill   the query lists of every slot.
 __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-// Work is grouped by (probe-rank band, list): band b holds ranks [2^b - 1, 2^(b+1) - 1),
-// so a query's nearest lists come first in the queue and its threshold is
-// tight when its farther lists are ranked.
-// Probe-rank band of rank p for band factor F = 2^shift: band b holds ranks
-// [(F^b - 1) / (F - 1), (F^(b+1) - 1) / (F - 1)), i.e. b = floor(log_F((F - 1) p + 1)).
-__device__ __forceinline__ uint32_t band_of(uint32_t p, uint32_t shift) {
-  return (31u - __clz(((1u << shift) - 1u) * p + 1u)) / shift;
+constexpr int INV_LT = 64;    // lists per tile
+constexpr int INV_QT = 256;   // queries per tile, one per thread
+constexpr int INV_MAXB = 16;  // bands at most (nprobe 4096, F = 2: 13)
+
+__device__ __forceinline__ uint64_t probe_key(const uint32_t *probes, const float *probe_d, size_t i) {
+  return (static_cast<uint64_t>(__float_as_uint(probe_d[i])) << 32) | probes[i];
 }
 
-__global__ void k_hist(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                       uint32_t np, uint32_t nlist, uint32_t shift, uint32_t *gcnt) {
-  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-  if (t >= static_cast<uint64_t>(B) * np) return;
-  const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
-  if (npq && p >= npq[q]) return;
-  atomicAdd(&gcnt[band_of(p, shift) * nlist + probes[t]], 1u);
+struct InvQuery {
+  uint64_t kb[INV_MAXB];  // kb[b]: key of the first probe of band b (b >= 1)
+  uint64_t kend;          // key of the last probe
+  bool valid;
+};
+
+__device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t q,
+                                     uint32_t B, uint32_t np, uint32_t nb, uint32_t shift) {
+  InvQuery r;
+  r.kend = 0;
+  r.valid = q < B;
+#pragma unroll
+  for (int b = 0; b < INV_MAXB; ++b) r.kb[b] = kNoKey;
+  if (q >= B) return r;
+  const uint32_t n = npq ? min(npq[q], np) : np;
+  const size_t base = static_cast<size_t>(q) * np;
+  r.kend = probe_key(probes, probe_d, base + n - 1);
+  const uint32_t F1 = (1u << shift) - 1u;
+#pragma unroll
+  for (int b = 1; b < INV_MAXB; ++b) {
+    if (static_cast<uint32_t>(b) >= nb) break;
+    const uint32_t sb = ((1u << (shift * b)) - 1u) / F1;
+    if (sb < n) r.kb[b] = probe_key(probes, probe_d, base + sb);
+  }
+  return r;
+}
+
+// Band of key k for the query, or -1 when the list is not probed.
+__device__ __forceinline__ int inv_band(const InvQuery &r, uint64_t k, uint32_t nb) {
+  if (!r.valid || k > r.kend) return -1;
+  int band = 0;
+#pragma unroll
+  for (int b = 1; b < INV_MAXB; ++b)
+    if (static_cast<uint32_t>(b) < nb) band += k >= r.kb[b] ? 1 : 0;
+  return band;
+}
+
+// Walks the tile's lists for the thread's query, calling f(list index in tile, band).
+template <class F>
+__device__ __forceinline__ void inv_walk(const float *__restrict__ dist, const InvQuery &r, uint32_t q, uint32_t B,
+                                         uint32_t nlist, uint32_t c0, uint32_t nb, F &&f) {
+  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
+  const float *row = dist + static_cast<size_t>(min(q, B - 1)) * nlist + c0;
+  const bool vec = (nlist % 4 == 0) && nt == INV_LT;
+  for (uint32_t i0 = 0; i0 < nt; i0 += 16) {
+    float v[16];
+    if (vec) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 x = *reinterpret_cast<const float4 *>(row + i0 + 4 * j);
+        v[4 * j] = x.x;
+        v[4 * j + 1] = x.y;
+        v[4 * j + 2] = x.z;
+        v[4 * j + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = i0 + j < nt ? row[i0 + j] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (i0 + j >= nt) break;
+      const uint64_t k = (static_cast<uint64_t>(__float_as_uint(v[j])) << 32) | (c0 + i0 + j);
+      f(i0 + j, inv_band(r, k, nb));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inv_count(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
+                                                   const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
+                                                   uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb,
+                                                   uint32_t shift, uint32_t *part) {
+  __shared__ uint32_t cnt[INV_LT * INV_MAXB];
+  const uint32_t c0 = blockIdx.x * INV_LT, qt = blockIdx.y, q = qt * INV_QT + threadIdx.x;
+  const uint32_t nslots = nb * nlist;
+  const int lane = lane_id();
+  for (int i = threadIdx.x; i < INV_LT * INV_MAXB; i += 256) cnt[i] = 0;
+  __syncthreads();
+  const InvQuery r = inv_query(probes, probe_d, npq, q, B, np, nb, shift);
+  inv_walk(dist, r, q, B, nlist, c0, nb, [&](uint32_t i, int band) {
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint64_t m = ballot(band == static_cast<int>(b));
+      if (m && lane == 0) atomicAdd(&cnt[i * INV_MAXB + b], static_cast<uint32_t>(__popcll(m)));
+    }
+  });
+  __syncthreads();
+  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
+  for (uint32_t e = threadIdx.x; e < nt * nb; e += 256) {
+    const uint32_t i = e / nb, b = e % nb;
+    part[static_cast<size_t>(qt) * nslots + b * nlist + c0 + i] = cnt[i * INV_MAXB + b];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inv_fill(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
+                                                  const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
+                                                  uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb, uint32_t shift,
+                                                  const uint32_t *__restrict__ part, const uint32_t *__restrict__ qoff,
+                                                  uint32_t *__restrict__ qlist) {
+  __shared__ uint32_t pos[INV_LT * INV_MAXB];
+  const uint32_t c0 = blockIdx.x * INV_LT, qt = blockIdx.y, q = qt * INV_QT + threadIdx.x;
+  const uint32_t nslots = nb * nlist;
+  const int lane = lane_id();
+  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
+  for (uint32_t e = threadIdx.x; e < nt * nb; e += 256) {
+    const uint32_t i = e / nb, b = e % nb, slot = b * nlist + c0 + i;
+    pos[i * INV_MAXB + b] = qoff[slot] + part[static_cast<size_t>(qt) * nslots + slot];
+  }
+  __syncthreads();
+  const InvQuery r = inv_query(probes, probe_d, npq, q, B, np, nb, shift);
+  const uint64_t lt = (1ull << lane) - 1ull;
+  inv_walk(dist, r, q, B, nlist, c0, nb, [&](uint32_t i, int band) {
+    for (uint32_t b = 0; b < nb; ++b) {
+      const bool mine = band == static_cast<int>(b);
+      const uint64_t m = ballot(mine);
+      if (!m) continue;
+      const int lead = __ffsll(static_cast<long long>(m)) - 1;
+      uint32_t base = 0;
+      if (lane == lead) base = atomicAdd(&pos[i * INV_MAXB + b], static_cast<uint32_t>(__popcll(m)));
+      base = __shfl(base, lead);
+      if (mine) qlist[base + __popcll(m & lt)] = q;
+    }
+  });
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -441,48 +611,76 @@ __device__ inline uint64_t block_exscan(uint64_t v, uint64_t *tmp, uint64_t *tot
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_plan_lists(const uint32_t *__restrict__ gcnt,
-                                                     const uint64_t *__restrict__ list_off, uint32_t nlist,
-                                                     uint32_t nslots, uint32_t qc, uint32_t rt, uint32_t *qoff,
-                                                     uint32_t *item_off, uint32_t *gfill, uint32_t *totals) {
+// Per slot: the tiles' offsets within it (in place), its queries and items;
+// per block of 1024 slots: their sums.
+__global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t qtiles, uint32_t nslots,
+                                                      uint32_t nlist, const uint64_t *__restrict__ list_off,
+                                                      uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *nit,
+                                                      uint32_t *bsum) {
   __shared__ uint64_t tmp[17];
-  const uint32_t per = (nslots + 1023) / 1024;
-  const uint32_t b = threadIdx.x * per, e = min(nslots, b + per);
-  uint64_t sg = 0, si = 0;
-  for (uint32_t c = b; c < e; ++c) {
-    const uint32_t g = gcnt[c];
-    const uint64_t n = list_off[c % nlist + 1] - list_off[c % nlist];
-    sg += g;
-    si += (g && n) ? static_cast<uint64_t>((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0;
+  const uint32_t sl = blockIdx.x * 1024 + threadIdx.x;
+  uint32_t g = 0, it = 0;
+  if (sl < nslots) {
+    for (uint32_t t = 0; t < qtiles; ++t) {
+      const size_t k = static_cast<size_t>(t) * nslots + sl;
+      const uint32_t v = part[k];
+      part[k] = g;
+      g += v;
+    }
+    const uint32_t c = sl % nlist;
+    const uint32_t n = static_cast<uint32_t>(list_off[c + 1] - list_off[c]);
+    it = (g && n) ? ((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0u;
+    gcnt[sl] = g;
+    nit[sl] = it;
   }
   uint64_t tg = 0, ti = 0;
-  uint64_t og = block_exscan(sg, tmp, &tg);
-  uint64_t oi = block_exscan(si, tmp, &ti);
-  for (uint32_t c = b; c < e; ++c) {
-    const uint32_t g = gcnt[c];
-    const uint64_t n = list_off[c % nlist + 1] - list_off[c % nlist];
-    qoff[c] = static_cast<uint32_t>(og);
-    item_off[c] = static_cast<uint32_t>(oi);
-    gfill[c] = 0;
-    og += g;
-    oi += (g && n) ? static_cast<uint64_t>((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0;
-  }
+  (void)block_exscan(g, tmp, &tg);
+  (void)block_exscan(it, tmp, &ti);
   if (threadIdx.x == 0) {
-    item_off[nslots] = static_cast<uint32_t>(ti);
-    totals[0] = static_cast<uint32_t>(tg);
-    totals[1] = static_cast<uint32_t>(ti);
+    bsum[2 * blockIdx.x] = static_cast<uint32_t>(tg);
+    bsum[2 * blockIdx.x + 1] = static_cast<uint32_t>(ti);
   }
 }
 
-__global__ void k_scatter(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq, uint32_t B,
-                          uint32_t np, uint32_t nlist, uint32_t shift, const uint32_t *__restrict__ qoff,
-                          uint32_t *gfill, uint32_t *qlist) {
-  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-  if (t >= static_cast<uint64_t>(B) * np) return;
-  const uint32_t q = static_cast<uint32_t>(t / np), p = static_cast<uint32_t>(t % np);
-  if (npq && p >= npq[q]) return;
-  const uint32_t c = band_of(p, shift) * nlist + probes[t];
-  qlist[qoff[c] + atomicAdd(&gfill[c], 1u)] = q;
+// Slot offsets into the query lists and item offsets (exclusive scans); the
+// last entry of item_off and totals = (pairs, items).
+__global__ __launch_bounds__(1024) void k_slot_scan(const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ nit,
+                                                    const uint32_t *__restrict__ bsum, uint32_t nslots, uint32_t *qoff,
+                                                    uint32_t *item_off, uint32_t *totals) {
+  __shared__ uint64_t tmp[17];
+  __shared__ uint32_t sbase[2];
+  const int tid = threadIdx.x, lane = lane_id();
+  if (tid < 64) {
+    uint64_t a = 0, b = 0;
+    for (uint32_t k = lane; k < blockIdx.x; k += 64) {
+      a += bsum[2 * k];
+      b += bsum[2 * k + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      b += __shfl_xor(b, o);
+    }
+    if (lane == 0) {
+      sbase[0] = static_cast<uint32_t>(a);
+      sbase[1] = static_cast<uint32_t>(b);
+    }
+  }
+  __syncthreads();
+  const uint32_t sl = blockIdx.x * 1024 + tid;
+  const uint32_t g = sl < nslots ? gcnt[sl] : 0u, it = sl < nslots ? nit[sl] : 0u;
+  uint64_t tg = 0, ti = 0;
+  const uint64_t og = block_exscan(g, tmp, &tg);
+  const uint64_t oi = block_exscan(it, tmp, &ti);
+  if (sl < nslots) {
+    qoff[sl] = sbase[0] + static_cast<uint32_t>(og);
+    item_off[sl] = sbase[1] + static_cast<uint32_t>(oi);
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+    item_off[nslots] = sbase[1] + static_cast<uint32_t>(ti);
+    totals[0] = sbase[0] + static_cast<uint32_t>(tg);
+    totals[1] = sbase[1] + static_cast<uint32_t>(ti);
+  }
 }
 
 // Per-batch resets of each query's pool and threshold (none yet: every
@@ -586,7 +784,7 @@ void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, u
 size_t select_smem(uint32_t nlist, uint32_t np) {
   uint32_t M = 1;
   while (M < np) M <<= 1;
-  return (nlist <= SEL_CACHE ? SEL_CACHE * 4 : 0) + static_cast<size_t>(M) * 8;
+  return (nlist <= SEL_CACHE ? static_cast<size_t>(nlist) * 4 : 0) + static_cast<size_t>(M) * 8;
 }
 
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
@@ -595,7 +793,8 @@ void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, u
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_select),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 4096);
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(select_smem(SEL_CACHE, SEL_MAXNP)));
     attr = true;
   }
   hipLaunchKernelGGL(k_select, dim3(B), dim3(256), sm, s, dist, B, nlist, np, probes, probe_d);
@@ -603,20 +802,28 @@ void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, u
 
 uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift) { return plan_bands(np, shift) * nlist; }
 
-void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
-                 uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
-                 uint64_t *thr, int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, hipStream_t s) {
-  const uint64_t t = static_cast<uint64_t>(B) * np;
-  const uint32_t g = static_cast<uint32_t>((t + 255) / 256);
-  const uint32_t nslots = plan_slots(nlist, np, shift);
-  hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, gcnt, nslots, 0u);
+size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift) {
+  const size_t nslots = plan_slots(nlist, np, shift);
+  const size_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
+  return qtiles * nslots + nslots + 2 * nblk;
+}
+
+void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
+                 uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
+                 uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
+                 uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
+                 uint32_t nstate, uint32_t shift, hipStream_t s) {
+  const uint32_t nb = plan_bands(np, shift), nslots = nb * nlist;
+  const uint32_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
+  uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
+  const dim3 tiles((nlist + INV_LT - 1) / INV_LT, qtiles);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
-  hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, shift, gcnt);
-  hipLaunchKernelGGL(k_plan_lists, dim3(1), dim3(1024), 0, s, gcnt, list_off, nlist, nslots, qc, rt, qoff, item_off,
-                     gfill, totals);
-  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, probes, npq, B, np, nlist, shift, qoff, gfill,
-                     qlist);
+  hipLaunchKernelGGL(k_inv_count, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift, part);
+  hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, gcnt,
+                     nit, bsum);
+  hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
+  hipLaunchKernelGGL(k_inv_fill, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift, part,
+                     qoff, qlist);
   hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr,
                      lbound);
 }

@@ -31,10 +31,15 @@ inline uint32_t plan_bands(uint32_t np, uint32_t shift) {
   return (31u - static_cast<uint32_t>(__builtin_clz(v))) / shift + 1u;
 }
 uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift);
-void launch_plan(const uint32_t *probes, const uint32_t *npq, uint32_t B, uint32_t np, const uint64_t *list_off,
-                 uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
-                 uint32_t *gfill, uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted,
-                 uint64_t *thr, int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, hipStream_t s);
+// 32-bit words of plan scratch for a batch (tile partial counts, item counts,
+// block sums).
+size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift);
+// Inversion and work plan from the coarse matrix and the sorted probes.
+void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
+                 uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
+                 uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
+                 uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
+                 uint32_t nstate, uint32_t shift, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
