@@ -119,7 +119,7 @@ def main():
         for _ in range(args.warmup):
             step(np_, S)
         torch.cuda.synchronize()
-        stage_sum = np.zeros(6)
+        stage_sum = np.zeros(len(katoann.STAGES))
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step(np_, S)
@@ -174,9 +174,12 @@ def main():
             "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"]},
         "qps_at_0999": b999["qps"] if b999 else None,
         "at_0999": {"nprobe": b999["nprobe"], "survivors": b999["survivors"]} if b999 else None,
-        "roofline": {"bound": "hbm", "kernel": "ranking pass", "achieved": head["rank_gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": head["rank_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": head["rank_bytes"], "ms_per_launch": head["stages_ms"]["rank"]},
+        "roofline": {"bound": "hbm", "kernel": "k_rank_band (the probe-rank bands of one search)",
+                     "achieved": head["rank_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["rank_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_search": head["rank_bytes"], "ms_per_search": head["stages_ms"]["rank"],
+                     "launches_per_search": head["work"]["bands"],
+                     "bytes_rule": "sum over touched lists of rows x (prefix bytes + 4-byte norm), once per search"},
         "stages_ms": head["stages_ms"],
         "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs")}
                   for r in sweep],

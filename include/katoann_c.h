@@ -134,11 +134,12 @@ int katoann_index_scale(const katoann_index *h, float *lo, double *inv_step, dou
  * and the ids in list order; any pointer may be NULL. */
 int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *centroid_codes, uint32_t *ids);
 
-/* Per-stage GPU times of the calling thread's last search on this index, in ms:
+/* Per-stage GPU times of the last search on this index, in ms:
  * [0] normalise + rotate + quantise, [1] coarse, [2] probe selection,
- * [3] inversion + plan, [4] ranking, [5] survivors + refinement + top-k.
+ * [3] inversion + plan, [4] ranking kernels (all bands), [5] survivor pool
+ * updates between bands, [6] survivors + refinement + top-k.
  * Timing is recorded only while enabled. */
-#define KATOANN_NSTAGES 6
+#define KATOANN_NSTAGES 7
 int katoann_set_timing(katoann_index *h, int32_t on);
 int katoann_stage_times(const katoann_index *h, float *ms, int32_t n);
 /* Work of the last search: [0] (query, list) pairs, [1] ranking items,

@@ -33,6 +33,7 @@ constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
 constexpr uint32_t kMaxSurvivors = 1024;
 constexpr uint32_t kPoolAllowance = 1024;  // pool slots per query beyond S for one band's kept keys
 constexpr uint32_t kMaxProbes = 4096;
+constexpr uint32_t kMaxBands = 16;  // probe-rank bands at most (nprobe 4096, factor 2: 13)
 
 thread_local std::string g_last_error;
 
@@ -80,16 +81,20 @@ void upload(DevBuf &b, const T *src, size_t count) {
 
 struct SearchCtx {
   hipStream_t stream = nullptr;
-  hipEvent_t ev[KATOANN_NSTAGES + 1] = {};
+  hipEvent_t ev[KATOANN_NSTAGES] = {};  // stage boundaries (the pool updates are timed as the rest of the ranking interval)
+  hipEvent_t bev[2 * kMaxBands] = {};   // around each band's ranking kernel
   DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
       totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items;
   std::vector<uint32_t> host_totals;
   SearchCtx() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    for (auto &e : bev) HIPCHECK(hipEventCreate(&e));
   }
   ~SearchCtx() {
     for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : bev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -450,6 +455,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.thr.ensure(B * 8ull);
   c.lbound.ensure(B * 4ull);
   const uint32_t nbands = katoann_hip::plan_bands(np, shift);
+  if (nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
   const uint32_t nstate = 3 + nbands;
   c.state.ensure(nstate * 4ull);
   const uint32_t rt = ix.rank_rows;
@@ -521,12 +527,14 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
     const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
     for (uint32_t band = 0; band < nbands; ++band) {
+      if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
       if (!katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, c.qhi.as<int8_t>(),
                                          dc, qn, c.lbound.as<int32_t>(), c.qlist.as<uint32_t>(),
                                          c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride, ix.step_sq,
                                          c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(),
                                          grid, s))
         throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+      if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band + 1], s));
       if (!katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
                                       c.thr.as<uint64_t>(), c.lbound.as<int32_t>(), B, S, stride, P, ix.step_sq, s))
         throw std::invalid_argument("unsupported survivor count");
@@ -579,12 +587,18 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   }
   {
     std::lock_guard<std::mutex> g(mix.stat_mu);
-    if (timing)
-      for (int i = 0; i < KATOANN_NSTAGES; ++i) {
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, c.ev[i], c.ev[i + 1]);
-        mix.stage_ms[i] = ms;
+    if (timing) {
+      float ms[6] = {};
+      for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]);
+      float rank = 0.0f;
+      for (uint32_t b = 0; b < nbands; ++b) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, c.bev[2 * b], c.bev[2 * b + 1]);
+        rank += t;
       }
+      const float out[KATOANN_NSTAGES] = {ms[0], ms[1], ms[2], ms[3], rank, std::max(0.0f, ms[4] - rank), ms[5]};
+      for (int i = 0; i < KATOANN_NSTAGES; ++i) mix.stage_ms[i] = out[i];
+    }
     mix.work[0] = st[4];  // (query, list) pairs
     mix.work[1] = st[5];  // ranking items
     mix.work[2] = st[1];  // candidates kept

@@ -25,8 +25,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libkatoann.so")
 
 RANKING = {"full": 0, "high": 1, "prefix": 2}
 SELECTION = {"threshold": 0, "local": 1}
-NSTAGES = 6
-STAGES = ("rotate+quantize", "coarse", "select", "plan", "rank", "refine")
+NSTAGES = 7
+STAGES = ("rotate+quantize", "coarse", "select", "plan", "rank", "update", "refine")
 
 
 class _BuildOptions(C.Structure):
