@@ -101,6 +101,25 @@ uint64_t katoann_memory_bytes(const katoann_index *h);
 
 /* ---- extensions (no reference counterpart) ----------------------------- */
 
+/* A trained learned-nprobe model (no reference counterpart): an MLP over each
+ * query's nfeat smallest sorted coarse distances (x_0 = d_0, x_i = d_i - d_0,
+ * standardised), two ReLU layers of `hidden`, one output z; the query's
+ * nprobe is ceil(2^(z * y_std + y_mean + margin)) clamped to [nmin, nprobe of
+ * the search]. Device pointers, owned by the caller; weights row-major
+ * (out x in), bf16. */
+typedef struct katoann_nprobe_model {
+  uint32_t nfeat, hidden;
+  const float *feat_mean, *feat_inv_std; /* nfeat */
+  const void *w1;                        /* hidden x nfeat */
+  const float *b1;                       /* hidden */
+  const void *w2;                        /* hidden x hidden */
+  const float *b2;                       /* hidden */
+  const void *w3;                        /* 1 x hidden */
+  const float *b3;                       /* 1 */
+  float y_mean, y_std, margin;
+  uint32_t nmin;
+} katoann_nprobe_model;
+
 /* Optional inputs and outputs of one search. */
 typedef struct katoann_search_extras {
   const uint32_t *nprobe_per_query; /* nq entries in [1, nprobe]: per-query probe count, or NULL */
@@ -113,6 +132,9 @@ typedef struct katoann_search_extras {
   float *rotated;                   /* nq x dim: rotated (normalised) queries */
   uint16_t *codes;                  /* nq x Dc: 16-bit query codes */
   void *stream;                     /* hipStream_t to run on (device_io only), or NULL for the index's own */
+  const katoann_nprobe_model *nprobe_model; /* per-query nprobe from the model (exclusive with nprobe_per_query) */
+  uint32_t *nprobe_used;            /* nq: the per-query nprobe the search used */
+  int32_t coarse_only;              /* stop after the probe selection (probes / probe_dists only) */
 } katoann_search_extras;
 
 int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
@@ -163,6 +185,28 @@ int katoann_sq16_l2(const uint8_t *qhi, const uint8_t *qlo, const uint8_t *xhi, 
 int katoann_sq16_high_l2(const uint8_t *qhi, const uint8_t *xhi, const int64_t *qnorm, const int64_t *xnorm,
                          double step_sq, uint32_t m, uint32_t n, uint32_t padded_dim, float *out,
                          uint32_t plane_dim);
+
+/* Kernels of the learned-nprobe MLP on device pointers (no reference
+ * counterpart), on `stream` (hipStream_t, NULL for the null stream).
+ * katoann_mlp_gemm: C[m][n] = act(sum_k A(m,k) B(n,k) + bias[n]), zero where
+ * gate[m][n] <= 0 when gate is given; A(m,k) = A[m sam + k sak] and
+ * B(n,k) = B[n sbn + k sbk] in bf16; C32 (fp32) and/or C16 (bf16), ldc apart. */
+int katoann_mlp_gemm(uint32_t M, uint32_t N, uint32_t K, const void *A, size_t sam, size_t sak, const void *B,
+                     size_t sbn, size_t sbk, const float *bias, const float *gate, size_t ldg, int32_t relu,
+                     float *C32, void *C16, size_t ldc, void *stream);
+int katoann_mlp_to_bf16(const float *x, void *y, uint64_t n, void *stream);
+/* grad = 2 (pred - y) / n; *loss += mean squared error (loss may be NULL) */
+int katoann_mlp_mse(const float *pred, const float *y, uint32_t n, float *grad, float *loss, void *stream);
+/* pinball loss at quantile tau: grad = -(tau - [y < pred]) / n; *loss += mean loss */
+int katoann_mlp_pinball(const float *pred, const float *y, uint32_t n, float tau, float *grad, float *loss,
+                        void *stream);
+int katoann_mlp_colsum(const float *g, uint32_t rows, uint32_t cols, float *out, void *stream);
+/* Adam step `step` (1-based) on fp32 weights; w16 (bf16 copy) may be NULL */
+int katoann_mlp_adam(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
+                     float eps, uint32_t step, void *w16, void *stream);
+/* Model inputs from sorted coarse distances (nq x nprobe): x32 and/or x16 (nq x nfeat) */
+int katoann_mlp_features(const float *probe_dists, uint32_t nq, uint32_t nprobe, uint32_t nfeat, const float *mean,
+                         const float *inv_std, float *x32, void *x16, void *stream);
 
 /* Row normalisation of the front ends (python/katoann_module.cpp:97-110) on the GPU. */
 int katoann_normalize_rows(float *rows, uint64_t n, uint32_t dim);

@@ -84,7 +84,8 @@ struct SearchCtx {
   hipEvent_t ev[KATOANN_NSTAGES] = {};  // stage boundaries (the pool updates are timed as the rest of the ranking interval)
   hipEvent_t bev[2 * kMaxBands] = {};   // around each band's ranking kernel
   DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
-      totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items;
+      totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items, mx16, mh1,
+      mh2, mz;
   std::vector<uint32_t> host_totals;
   SearchCtx() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -473,6 +474,18 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   if (x && x->survivors) c.surv.ensure(static_cast<size_t>(B) * S * 8);
   if (x && x->n_survivors) c.nsurv.ensure(B * 4ull);
   if (x && x->codes) c.codes.ensure(static_cast<size_t>(B) * dc * 2);
+  const katoann_nprobe_model *model = x ? x->nprobe_model : nullptr;
+  const bool coarse_only = x && x->coarse_only;
+  if (model) {
+    if (x->nprobe_per_query) throw std::invalid_argument("nprobe_model and nprobe_per_query are exclusive");
+    if (model->nfeat == 0 || model->nfeat > np || model->hidden == 0)
+      throw std::invalid_argument("nprobe model needs 1 <= nfeat <= nprobe and hidden >= 1");
+    c.npq.ensure(B * 4ull);
+    c.mx16.ensure(static_cast<size_t>(B) * model->nfeat * 2);
+    c.mh1.ensure(static_cast<size_t>(B) * model->hidden * 2);
+    c.mh2.ensure(static_cast<size_t>(B) * model->hidden * 2);
+    c.mz.ensure(B * 4ull);
+  }
   const uint32_t *npq = nullptr;
   if (x && x->nprobe_per_query) {
     c.npq.ensure(B * 4ull);
@@ -513,7 +526,29 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     mark(2);
     katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
                                s);
+    if (model) {  // per-query nprobe from the sorted coarse distances
+      const uint32_t F = model->nfeat, H = model->hidden;
+      katoann_hip::launch_features(c.probe_d.as<float>(), B, np, F, model->feat_mean, model->feat_inv_std, nullptr,
+                                   c.mx16.p, s);
+      katoann_hip::launch_gemm_bf16(B, H, F, c.mx16.p, F, 1, model->w1, F, 1, model->b1, nullptr, 0, 1, nullptr,
+                                    c.mh1.p, H, s);
+      katoann_hip::launch_gemm_bf16(B, H, H, c.mh1.p, H, 1, model->w2, H, 1, model->b2, nullptr, 0, 1, nullptr,
+                                    c.mh2.p, H, s);
+      katoann_hip::launch_gemm_bf16(B, 1, H, c.mh2.p, H, 1, model->w3, H, 1, model->b3, nullptr, 0, 0,
+                                    c.mz.as<float>(), nullptr, 1, s);
+      katoann_hip::launch_nprobe_of(c.mz.as<float>(), B, model->y_mean, model->y_std, model->margin,
+                                    std::max(1u, std::min(model->nmin, np)), np, c.npq.as<uint32_t>(), s);
+      npq = c.npq.as<uint32_t>();
+    }
     mark(3);
+    if (coarse_only) {
+      mark(4);
+      mark(5);
+      mark(6);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(s));
+      return true;
+    }
     katoann_hip::launch_plan(c.dist.as<float>(), c.probes.as<uint32_t>(), c.probe_d.as<float>(), npq, B, np,
                              ix.list_off_d.as<uint64_t>(), nlist, qc, rt, c.pscratch.as<uint32_t>(),
                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
@@ -564,7 +599,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   }
 
   const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (!dev_io) {
+  if (!dev_io && !coarse_only) {
     HIPCHECK(hipMemcpyAsync(out_ids, dids, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(out_dists, ddst, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
   }
@@ -576,11 +611,25 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     if (x->n_survivors) HIPCHECK(hipMemcpyAsync(x->n_survivors, c.nsurv.p, B * 4ull, d2o, s));
     if (x->rotated) HIPCHECK(hipMemcpyAsync(x->rotated, yq, static_cast<size_t>(B) * d * 4, d2o, s));
     if (x->codes) HIPCHECK(hipMemcpyAsync(x->codes, c.codes.p, static_cast<size_t>(B) * dc * 2, d2o, s));
+    if (x->nprobe_used) {
+      if (npq) {
+        HIPCHECK(hipMemcpyAsync(x->nprobe_used, npq, B * 4ull, d2o, s));
+      } else {
+        HIPCHECK(hipStreamSynchronize(s));
+        std::vector<uint32_t> v(B, np);
+        HIPCHECK(hipMemcpyAsync(x->nprobe_used, v.data(), B * 4ull,
+                                dev_io ? hipMemcpyHostToDevice : hipMemcpyHostToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+      }
+    }
   }
   HIPCHECK(hipStreamSynchronize(s));
-  if (x && npq && !dev_io) {  // past a query's own count: no list
+  if (x && npq && !dev_io && (x->probes || x->probe_dists)) {  // past a query's own count: no list
+    std::vector<uint32_t> hn(B);
+    if (x->nprobe_per_query) std::copy(x->nprobe_per_query, x->nprobe_per_query + B, hn.begin());
+    else HIPCHECK(hipMemcpy(hn.data(), npq, B * 4ull, hipMemcpyDeviceToHost));
     for (uint32_t q = 0; q < B; ++q)
-      for (uint32_t j = x->nprobe_per_query[q]; j < np; ++j) {
+      for (uint32_t j = hn[q]; j < np; ++j) {
         if (x->probes) x->probes[static_cast<size_t>(q) * np + j] = UINT32_MAX;
         if (x->probe_dists) x->probe_dists[static_cast<size_t>(q) * np + j] = std::numeric_limits<float>::max();
       }
@@ -807,6 +856,66 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
   if (log2_factor < 1 || log2_factor > 4) throw std::invalid_argument("band factor must be 2, 4, 8 or 16");
   std::lock_guard<std::mutex> g(h->mu);
   h->band_shift = log2_factor;
+  KATOANN_CATCH
+}
+
+int katoann_mlp_gemm(uint32_t M, uint32_t N, uint32_t K, const void *A, size_t sam, size_t sak, const void *B,
+                     size_t sbn, size_t sbk, const float *bias, const float *gate, size_t ldg, int32_t relu,
+                     float *C32, void *C16, size_t ldc, void *stream) {
+  KATOANN_TRY
+  if (!A || !B || (!C32 && !C16)) throw std::invalid_argument("gemm needs A, B and an output");
+  katoann_hip::launch_gemm_bf16(M, N, K, A, sam, sak, B, sbn, sbk, bias, gate, ldg, relu, C32, C16, ldc,
+                                static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_to_bf16(const float *x, void *y, uint64_t n, void *stream) {
+  KATOANN_TRY
+  katoann_hip::launch_to_bf16(x, y, n, static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_mse(const float *pred, const float *y, uint32_t n, float *grad, float *loss, void *stream) {
+  KATOANN_TRY
+  katoann_hip::launch_mse(pred, y, n, grad, loss, static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_pinball(const float *pred, const float *y, uint32_t n, float tau, float *grad, float *loss,
+                        void *stream) {
+  KATOANN_TRY
+  if (!(tau > 0.0f && tau < 1.0f)) throw std::invalid_argument("pinball quantile must be in (0, 1)");
+  katoann_hip::launch_pinball(pred, y, n, tau, grad, loss, static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_colsum(const float *g, uint32_t rows, uint32_t cols, float *out, void *stream) {
+  KATOANN_TRY
+  katoann_hip::launch_colsum(g, rows, cols, out, static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_adam(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
+                     float eps, uint32_t step, void *w16, void *stream) {
+  KATOANN_TRY
+  if (step == 0) throw std::invalid_argument("Adam steps count from 1");
+  katoann_hip::launch_adam(w, g, m, v, n, lr, b1, b2, eps, step, w16, static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
+  KATOANN_CATCH
+}
+
+int katoann_mlp_features(const float *probe_dists, uint32_t nq, uint32_t nprobe, uint32_t nfeat, const float *mean,
+                         const float *inv_std, float *x32, void *x16, void *stream) {
+  KATOANN_TRY
+  if (nfeat == 0 || nfeat > nprobe) throw std::invalid_argument("need 1 <= nfeat <= nprobe");
+  katoann_hip::launch_features(probe_dists, nq, nprobe, nfeat, mean, inv_std, x32, x16,
+                               static_cast<hipStream_t>(stream));
+  HIPCHECK(hipGetLastError());
   KATOANN_CATCH
 }
 

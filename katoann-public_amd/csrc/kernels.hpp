@@ -62,4 +62,20 @@ void launch_sq16_pairs(const int8_t *qh, const int8_t *ql, const int64_t *qn, co
                        const int64_t *xn, uint32_t m, uint32_t n, uint32_t dc, uint32_t qstride, uint32_t xstride,
                        int full, double step_sq, float *out, hipStream_t s);
 
+// Learned nprobe (mlp.hip).
+void launch_gemm_bf16(uint32_t M, uint32_t N, uint32_t K, const void *A, size_t sam, size_t sak, const void *B,
+                      size_t sbn, size_t sbk, const float *bias, const float *gate, size_t ldg, int relu, float *C32,
+                      void *C16, size_t ldc, hipStream_t s);
+void launch_to_bf16(const float *x, void *y, uint64_t n, hipStream_t s);
+void launch_mse(const float *p, const float *y, uint32_t n, float *grad, float *loss, hipStream_t s);
+void launch_pinball(const float *p, const float *y, uint32_t n, float tau, float *grad, float *loss,
+                    hipStream_t s);
+void launch_colsum(const float *g, uint32_t R, uint32_t C, float *out, hipStream_t s);
+void launch_adam(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2, float eps,
+                 uint32_t step, void *w16, hipStream_t s);
+void launch_features(const float *pd, uint32_t B, uint32_t np, uint32_t F, const float *mean, const float *inv_std,
+                     float *x32, void *x16, hipStream_t s);
+void launch_nprobe_of(const float *z, uint32_t B, float y_mean, float y_std, float margin, uint32_t nmin,
+                      uint32_t nmax, uint32_t *npq, hipStream_t s);
+
 }  // namespace katoann_hip

@@ -45,11 +45,20 @@ class _SearchParams(C.Structure):
                 ("survivors", C.c_uint32), ("selection", C.c_int32)]
 
 
+class NprobeModel(C.Structure):
+    """katoann_nprobe_model: a trained learned-nprobe MLP (device pointers)."""
+    _fields_ = [("nfeat", C.c_uint32), ("hidden", C.c_uint32), ("feat_mean", C.c_void_p),
+                ("feat_inv_std", C.c_void_p), ("w1", C.c_void_p), ("b1", C.c_void_p), ("w2", C.c_void_p),
+                ("b2", C.c_void_p), ("w3", C.c_void_p), ("b3", C.c_void_p), ("y_mean", C.c_float),
+                ("y_std", C.c_float), ("margin", C.c_float), ("nmin", C.c_uint32)]
+
+
 class _Extras(C.Structure):
     _fields_ = [("nprobe_per_query", C.c_void_p), ("normalize", C.c_int32), ("device_io", C.c_int32),
                 ("probes", C.c_void_p), ("probe_dists", C.c_void_p), ("survivors", C.c_void_p),
                 ("n_survivors", C.c_void_p), ("rotated", C.c_void_p), ("codes", C.c_void_p),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("nprobe_model", C.POINTER(NprobeModel)), ("nprobe_used", C.c_void_p),
+                ("coarse_only", C.c_int32)]
 
 
 _lib = None
@@ -61,6 +70,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libkatoann.so is not built: run python katoann-public_amd/build.py")
+        # One HIP runtime per process. torch ships its own libamdhip64 and asks
+        # for it as "libamdhip64.so", which does not match the "libamdhip64.so.7"
+        # this library binds; loaded in the other order, the process holds two
+        # runtimes and torch's finds no GPU. Loading torch first lets this
+        # library bind to torch's runtime (same soname).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P, u32, u64, i32, f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
         L.katoann_last_error.restype = C.c_char_p
@@ -89,6 +107,14 @@ def lib():
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
         L.katoann_rotate_rows.argtypes = [P, P, u64, u32, P]
         L.katoann_build.argtypes = [P, u32, u32, C.POINTER(_BuildOptions), C.c_char_p]
+        sz, f32 = C.c_size_t, C.c_float
+        L.katoann_mlp_gemm.argtypes = [u32, u32, u32, P, sz, sz, P, sz, sz, P, P, sz, i32, P, P, sz, P]
+        L.katoann_mlp_to_bf16.argtypes = [P, P, u64, P]
+        L.katoann_mlp_mse.argtypes = [P, P, u32, P, P, P]
+        L.katoann_mlp_colsum.argtypes = [P, u32, u32, P, P]
+        L.katoann_mlp_pinball.argtypes = [P, P, u32, f32, P, P, P]
+        L.katoann_mlp_adam.argtypes = [P, P, P, P, u32, f32, f32, f32, f32, u32, P, P]
+        L.katoann_mlp_features.argtypes = [P, u32, u32, u32, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -247,16 +273,26 @@ class Index:
 
     def search_device(self, q_ptr: int, nq: int, ids_ptr: int, dists_ptr: int, k=10, nprobe=64,
                       ranking="prefix", survivors=32, selection="threshold", npq_ptr: int | None = None,
-                      normalize=False, stream: int | None = None):
-        """Search with queries and outputs in device memory (e.g. torch tensors' data_ptr())."""
+                      normalize=False, stream: int | None = None, model: NprobeModel | None = None,
+                      npq_out_ptr: int | None = None, probes_ptr: int | None = None,
+                      probe_dists_ptr: int | None = None, coarse_only: bool = False):
+        """Search with queries and outputs in device memory (e.g. torch tensors' data_ptr()).
+        model: a learned-nprobe model (per-query nprobe in [model.nmin, nprobe]);
+        coarse_only: stop after the probe selection (probes / probe dists only)."""
         p = self._params(k, nprobe, ranking, survivors, selection)
         x = _Extras()
         x.device_io = 1
         x.normalize = int(normalize)
         x.nprobe_per_query = npq_ptr
         x.stream = stream
+        x.nprobe_used = npq_out_ptr
+        x.probes = probes_ptr
+        x.probe_dists = probe_dists_ptr
+        x.coarse_only = int(coarse_only)
+        if model is not None:
+            x.nprobe_model = C.pointer(model)
         _check(lib().katoann_search_ex(self._h, C.c_void_p(q_ptr), nq, C.byref(p), C.byref(x),
-                                       C.c_void_p(ids_ptr), C.c_void_p(dists_ptr)))
+                                       C.c_void_p(ids_ptr or 0), C.c_void_p(dists_ptr or 0)))
 
     def load_arrays(self, dim, nlist, n, normalized, centroids, list_sizes, ids, vectors, rotation,
                     prefix_dims, vectors_device_ptr: int | None = None):

@@ -141,3 +141,29 @@ def build_torch(data, prefix: str | None, nlist: int, train_size: int = 256000,
         formats.write_index(prefix, cen.cpu().numpy(), lists, [host[ids] for ids in lists], n,
                             normalized, rot.cpu().numpy())
     return out
+
+
+def load_torch(data, nlist: int, train_size: int = 256000, kmeans_iters: int = 10, seed: int = 1,
+               normalized: bool = True):
+    """Builds an index from a GPU tensor with build_torch and loads it from
+    device memory (no files). Returns (Index, info) where info holds the
+    build dict plus ``sizes`` (numpy) and ``list_of_id`` (device int32: the
+    list of every base id)."""
+    import torch
+    from . import Index, formats
+    b = build_torch(data, None, nlist=nlist, train_size=train_size, kmeans_iters=kmeans_iters, seed=seed,
+                    normalized=normalized)
+    order = b["order"]
+    sizes = b["sizes"].cpu().numpy().astype(np.uint32)
+    rows = data.index_select(0, order).contiguous()
+    ix = Index()
+    ix.load_arrays(data.shape[1], nlist, data.shape[0], normalized, b["centroids"].cpu().numpy(), sizes,
+                   order.cpu().numpy().astype(np.uint32), None, b["rotation"].cpu().numpy(),
+                   formats.default_prefix_dims(data.shape[1]), vectors_device_ptr=rows.data_ptr())
+    torch.cuda.synchronize()
+    del rows
+    list_of_id = torch.empty(data.shape[0], dtype=torch.int32, device=data.device)
+    list_of_id[order] = torch.repeat_interleave(torch.arange(nlist, dtype=torch.int32, device=data.device),
+                                                b["sizes"].to(data.device).long())
+    info = dict(b, sizes=sizes, list_of_id=list_of_id)
+    return ix, info

@@ -102,6 +102,22 @@ def spectrum_torch(n: int, nq: int, d: int, alpha: float = 1.0, seed: int = 2, d
     return draw(n), draw(nq)
 
 
+def spectrum_queries_torch(nq: int, d: int, alpha: float = 1.0, seed: int = 2, draw_seed: int = 101,
+                           device="cuda", chunk: int = 262144):
+    """More queries of spectrum_torch(..., seed)'s law (same basis and
+    spectrum), drawn from their own seeded generator."""
+    import torch
+    q = random_basis_torch(d, _torch_gen(seed, device), device).float()
+    g = _torch_gen(draw_seed, device)
+    sl = torch.sqrt(torch.arange(1, d + 1, dtype=torch.float64, device=device) ** (-alpha)).float()
+    out = torch.empty((nq, d), dtype=torch.float32, device=device)
+    for s in range(0, nq, chunk):
+        e = min(nq, s + chunk)
+        z = torch.randn((e - s, d), generator=g, device=device, dtype=torch.float32)
+        out[s:e] = _normalize_torch((z * sl) @ q.T)
+    return out
+
+
 def zipf_mixture_torch(n: int, nq: int, d: int = 1024, comps: int = 16384, s: float = 1.1,
                        noise: float = 0.5, seed: int = 3, device="cuda", chunk: int = 262144):
     """Component weights ~ j^-s; means and noise share the i^-1 spectrum in one basis."""
