@@ -63,6 +63,12 @@ def main():
     ap.add_argument("--band-shift", type=int, default=0, help="probe-rank bands grow by 2**shift (default: library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1024)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-train", type=int, default=50000, help="training queries of the learned nprobe")
+    ap.add_argument("--learned-nmax", type=int, default=2048)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
+    ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (--gpus 1)", file=sys.stderr)
@@ -150,6 +156,21 @@ def main():
     b99, b999 = best(0.99), best(0.999)
     head = b99 or max(sweep, key=lambda r: r["recall"])
 
+    learned_out, learned_model = None, None
+    if not args.no_learned and b99:
+        try:
+            learned_out, learned_model = learned_leg(ix, base, queries, gt, b, sizes, args, b99, ids_d, dists_d)
+        except Exception as e:  # reported, never silently replaced
+            log("learned leg failed: %r" % (e,))
+
+    serve_out = None
+    if not args.no_serve:
+        try:
+            serve_out = serve_leg(ix, queries, args, head, b999 or max(sweep, key=lambda r: r["recall"]),
+                                  learned_model, learned_out)
+        except Exception as e:  # reported, never silently replaced
+            log("serving leg failed: %r" % (e,))
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -183,9 +204,139 @@ def main():
         "stages_ms": head["stages_ms"],
         "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs")}
                   for r in sweep],
+        "learned_nprobe": learned_out,
+        "serving": serve_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_leg(ix, base, queries, gt, b, sizes, args, plain, ids_d, dists_d):
+    """Per-query nprobe from the MLP trained on the GPU (untimed): training
+    queries and validation queries are drawn apart from the benchmark queries;
+    the margin is the smallest reaching recall@10 0.99 on the validation set;
+    then the benchmark batch is timed like the fixed-nprobe sweep."""
+    import torch
+    from katoann import datasets, learned
+    dev = queries.device
+    n, d, nlist, nq, S = base.shape[0], base.shape[1], args.nlist, args.nq, plain["survivors"]
+    t0 = time.time()
+    list_of_id = torch.empty(n, dtype=torch.int32, device=dev)
+    list_of_id[b["order"].long()] = torch.repeat_interleave(
+        torch.arange(nlist, dtype=torch.int32, device=dev), torch.from_numpy(sizes.astype(np.int64)).to(dev))
+    trq = datasets.spectrum_queries_torch(args.learned_train, d, alpha=args.alpha, seed=2, draw_seed=11, device=dev)
+    gtr = torch.from_numpy(datasets.ground_truth_torch(base, trq, 10).astype(np.int64)).to(dev)
+    model, labels, hist = learned.train(ix, trq, list_of_id, gtr, epochs=20, seed=0)
+    del trq, gtr
+    val = datasets.spectrum_queries_torch(nq, d, alpha=args.alpha, seed=2, draw_seed=12, device=dev)
+    gval = datasets.ground_truth_torch(base, val, 10)
+    torch.cuda.synchronize()
+    log("learned: training set + fit %.1fs, loss %.3f -> %.3f" % (time.time() - t0, hist[0], hist[-1]))
+    nmax = min(args.learned_nmax, nlist)
+    used = torch.empty(nq, dtype=torch.int32, device=dev)
+
+    def run(qt, margin):
+        ix.search_device(qt.data_ptr(), nq, ids_d.data_ptr(), dists_d.data_ptr(), k=args.k, nprobe=nmax,
+                         survivors=S, model=model.struct(margin), npq_out_ptr=used.data_ptr())
+
+    margin = None
+    for mg in np.arange(-0.5, 4.01, 0.125):
+        run(val, float(mg))
+        if recall_at(ids_d.cpu().numpy().view(np.uint32), gval, 10) >= 0.99:
+            margin = float(mg)
+            break
+    if margin is None:
+        raise RuntimeError("no margin reaches recall 0.99 on the validation queries")
+    for _ in range(args.warmup):
+        run(queries, margin)
+    torch.cuda.synchronize()
+    ix.set_timing(True)
+    stage_sum = np.zeros(len(__import__("katoann").STAGES))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run(queries, margin)
+        stage_sum += np.array(list(ix.stage_times().values()))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ix.set_timing(False)
+    rec = recall_at(ids_d.cpu().numpy().view(np.uint32), gt, 10)
+    mean_np = float(used.float().mean().item())
+    import katoann
+    out = {"qps": nq / (ms / 1e3), "ms_per_step": ms, "recall": rec, "margin": margin, "survivors": S,
+           "nprobe_max": nmax, "mean_nprobe": mean_np, "plain_nprobe": plain["nprobe"],
+           "plain_qps": plain["qps"], "speedup_vs_plain": (nq / (ms / 1e3)) / plain["qps"],
+           "stages_ms": dict(zip(katoann.STAGES, (stage_sum / args.steps).tolist())),
+           "training": "%d queries of the same law (own seed), coverage labels from exact top-10, 20 epochs "
+                       "Adam, bf16 MFMA; margin calibrated on %d further queries" % (args.learned_train, nq)}
+    log("learned: " + json.dumps(out))
+    return out, (model, model.struct(margin), nmax)
+
+
+def serve_leg(ix, queries, args, at99, at999, learned_model, learned_out):
+    """Continuous batching under Poisson arrivals: request sizes floor(exp(U(0,
+    ln 4096))) queries drawn from the benchmark queries, k in {10, 100}, target
+    recall@10 0.99 (learned nprobe when trained, else the fixed nprobe reaching
+    0.99) or 0.999 (the fixed nprobe reaching it, else the sweep's highest
+    recall); offered load = serve_load x the batch QPS at 0.99. Latency runs
+    from submission (at the scheduled arrival) to results ready."""
+    import threading
+    import katoann
+    host_q = queries.cpu().numpy()
+    rng = np.random.default_rng(42)
+    mean_size = (4096 - 1) / np.log(4096)
+    rate = args.serve_load * at99["qps"] / mean_size  # requests per second
+    n_req = max(1, int(rate * args.serve_seconds))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    sizes = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4096), n_req)))).astype(int)
+    ks = rng.choice([10, 100], n_req)
+    hi = rng.random(n_req) < 0.5
+    starts = rng.integers(0, host_q.shape[0] - sizes + 1)
+    tickets = [None] * n_req
+    lag = [0.0]
+    srv = katoann.Server(ix, max_inflight=4, max_batch=4096)
+
+    def params(i):
+        a = at999 if hi[i] else at99
+        return dict(k=int(ks[i]), nprobe=a["nprobe"], survivors=max(a["survivors"], int(ks[i])))
+
+    def producer():
+        t0 = time.perf_counter()
+        t = 0.0
+        for i in range(n_req):
+            t += gaps[i]
+            now = time.perf_counter() - t0
+            if now < t:
+                time.sleep(t - now)
+            else:
+                lag[0] = max(lag[0], now - t)
+            p = params(i)
+            model = None
+            if not hi[i] and learned_model is not None:
+                model = learned_model[1]
+                p["nprobe"] = learned_model[2]
+                p["survivors"] = max(learned_out["survivors"], p["k"])
+            tickets[i] = srv.submit(host_q[starts[i]:starts[i] + sizes[i]], model=model, normalize=False, **p)
+
+    th = threading.Thread(target=producer)
+    t0 = time.perf_counter()
+    th.start()
+    th.join()
+    srv.drain()
+    wall = time.perf_counter() - t0
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    srv.close()
+    out = {"requests": n_req, "queries": int(sizes.sum()), "offered_qps": float(sizes.sum() / gaps.sum()),
+           "achieved_qps": st["achieved_qps"], "p50_ms": st["p50_ms"], "p95_ms": st["p95_ms"],
+           "p99_ms": st["p99_ms"], "max_ms": st["max_ms"], "mean_batch": st["mean_batch"],
+           "batches": st["batches"], "peak_inflight": st["peak_inflight"], "peak_reserved_bytes": st["peak_reserved"],
+           "budget_bytes": st["budget"], "wall_s": wall, "max_submit_lag_ms": lag[0] * 1e3,
+           "mix": "sizes floor(exp(U(0, ln 4096))), k {10, 100}, half at target 0.99 (%s), half at 0.999 "
+                  "(nprobe %d, survivors %d)" % ("learned nprobe" if learned_model is not None else
+                                                 "nprobe %d" % at99["nprobe"], at999["nprobe"], at999["survivors"])}
+    log("serving: " + json.dumps(out))
+    return out
 
 
 def cpu_baseline(base, queries, b, sizes, pd, args, nprobe, survivors):

@@ -208,6 +208,58 @@ int katoann_mlp_adam(float *w, const float *g, float *m, float *v, uint32_t n, f
 int katoann_mlp_features(const float *probe_dists, uint32_t nq, uint32_t nprobe, uint32_t nfeat, const float *mean,
                          const float *inv_std, float *x32, void *x16, void *stream);
 
+/* Device bytes one search of nq queries with these parameters and extras
+ * allocates (its per-search buffers at the first pool allowance), after the
+ * same argument checks as katoann_search_ex. */
+int katoann_search_footprint(const katoann_index *h, uint32_t nq, const katoann_search_params *p,
+                             const katoann_search_extras *x, uint64_t *bytes);
+
+/* ---- continuous-batching server (no reference counterpart) --------------
+ * Replaces the synchronous idle batching of the CLI's `load` command
+ * (tools/katoann.cpp:267-337): requests (host queries, any size, each with its
+ * own k / nprobe / survivors / model) queue as they arrive; up to
+ * max_inflight batches run at once, each on its own HIP stream, so that their
+ * kernels overlap on the GPU. A batch is the queue's oldest request plus the
+ * following requests with the same parameters, up to max_batch queries. It is
+ * admitted when a slot is free and its search footprint fits the HBM budget
+ * beside the batches in flight; a request that can never fit is refused at
+ * submission. Each query's result does not depend on the batch it ran in. */
+typedef struct katoann_server katoann_server;
+
+typedef struct katoann_server_options {
+  uint32_t max_inflight; /* batches on the GPU at once (default 4) */
+  uint32_t max_batch;    /* queries per batch (default 4096) */
+  uint64_t hbm_budget;   /* bytes for search buffers; 0: 90% of the free HBM at creation */
+} katoann_server_options;
+
+typedef struct katoann_server_stats {
+  uint64_t requests, queries, batches; /* completed */
+  double p50_ms, p95_ms, p99_ms, max_ms; /* arrival (submission) to results ready; sorted[min(n-1, ceil(q n)-1)] */
+  double mean_batch;                   /* queries per batch */
+  double achieved_qps;                 /* queries / (last completion - first arrival) */
+  uint64_t budget, peak_reserved;      /* bytes */
+  uint32_t peak_inflight;
+} katoann_server_stats;
+
+void katoann_default_server_options(katoann_server_options *o);
+/* The index must outlive the server. */
+int katoann_server_create(katoann_index *h, const katoann_server_options *o, katoann_server **out);
+/* Drains the queue, then stops. */
+void katoann_server_destroy(katoann_server *s);
+/* Queues nq x dim host queries (copied; normalised if normalize) for a search
+ * with p and an optional model (copied; its device weights must stay valid
+ * until the request completes); *ticket names the request. */
+int katoann_server_submit(katoann_server *s, const float *queries, uint32_t nq, const katoann_search_params *p,
+                          const katoann_nprobe_model *model, int32_t normalize, uint64_t *ticket);
+/* *done = 1 once the request's results are ready (or it failed). */
+int katoann_server_poll(katoann_server *s, uint64_t ticket, int32_t *done);
+/* Waits for the request, writes nq x k ids and dists (host) and its latency in
+ * ms (may be NULL), and forgets the ticket; returns the search's error, if any. */
+int katoann_server_wait(katoann_server *s, uint64_t ticket, uint32_t *ids, float *dists, double *latency_ms);
+/* Waits until no request is queued or running. */
+int katoann_server_drain(katoann_server *s);
+int katoann_server_stats_get(katoann_server *s, katoann_server_stats *out);
+
 /* Row normalisation of the front ends (python/katoann_module.cpp:97-110) on the GPU. */
 int katoann_normalize_rows(float *rows, uint64_t n, uint32_t dim);
 

@@ -401,9 +401,23 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 
 // ---------------------------------------------------------------- search
 
-void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
-                 const katoann_search_extras *x, uint32_t *out_ids, float *out_dists) {
-  if (nq == 0) return;
+// Validated shape of one search and the bytes of every per-search buffer, so
+// that the search and the admission estimate of a serving loop agree.
+struct SearchPlan {
+  uint32_t B, d, dc, np, K, S, nlist, P, shift, qc, nslots, nbands, nstate, rt, item_cap;
+  bool prefix;
+  uint64_t max_rows, allowance;
+  size_t q, qcodes, qnorm, q4, dist, probes, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
+      codes, npq, mx16, mh, pool;
+  size_t total() const {
+    return 2 * q + 2 * qcodes + qnorm + 2 * q4 + dist + 3 * probes + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
+           q4 /* pool_sorted */ + thr + q4 /* lbound */ + state + items + 2 * out + surv + (surv ? q4 : 0) + codes +
+           npq + mx16 + 2 * mh + (mx16 ? q4 : 0) + pool;
+  }
+};
+
+SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_search_params &p,
+                       const katoann_search_extras *x) {
   if (ix.nlist == 0) throw logic_err("search before load");
   if (p.k == 0 || p.nprobe == 0 || p.nprobe > ix.nlist)
     throw std::invalid_argument("need k >= 1 and 1 <= nprobe <= nlist");
@@ -415,15 +429,72 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     throw std::invalid_argument("ranking is 'prefix', 'high' or 'full'");
   if (p.selection != KATOANN_SEL_THRESHOLD && p.selection != KATOANN_SEL_LOCAL)
     throw std::invalid_argument("selection is 'threshold' or 'local'");
-  const uint32_t S = std::max(p.survivors, p.k);
-  if (S > kMaxSurvivors)
+  SearchPlan z{};
+  z.S = std::max(p.survivors, p.k);
+  if (z.S > kMaxSurvivors)
     throw std::invalid_argument("max(survivors, k) above 1024 is not supported by the gfx950 engine");
   if (p.nprobe > kMaxProbes) throw std::invalid_argument("nprobe above 4096 is not supported by the gfx950 engine");
-  const bool prefix = p.ranking == KATOANN_RANK_PREFIX;
-  const uint32_t P = prefix ? ix.head_dims : ix.dc;
-  if (P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
+  z.prefix = p.ranking == KATOANN_RANK_PREFIX;
+  z.P = z.prefix ? ix.head_dims : ix.dc;
+  if (z.P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
+  z.B = nq, z.d = ix.dim, z.dc = ix.dc, z.np = p.nprobe, z.K = p.k, z.nlist = ix.nlist;
+  z.shift = ix.band_shift;
+  z.qc = katoann_hip::rank_queries_per_item(z.P);
+  if (z.qc == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+  z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift);
+  z.nbands = katoann_hip::plan_bands(z.np, z.shift);
+  if (z.nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
+  z.nstate = 3 + z.nbands;
+  z.rt = ix.rank_rows;
+  auto &mix = const_cast<katoann_index &>(ix);
+  z.max_rows = mix.max_rows_per_query(z.np);
+  // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
+  // groups over all bands hold at most B queries
+  uint64_t tiles_total = 0;
+  for (uint32_t cc = 0; cc < z.nlist; ++cc) tiles_total += (ix.sizes[cc] + z.rt - 1) / z.rt;
+  const uint64_t item_cap64 = tiles_total * ((z.B + z.qc - 1) / z.qc + z.nbands);
+  if (item_cap64 > 0x7fffffffull) throw std::invalid_argument("batch too large for the ranking plan");
+  z.item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
+  // the first bands run without a threshold (or with one from few candidates):
+  // room for every row of a query's 1 + F nearest lists
+  z.allowance = std::min(z.max_rows, std::max<uint64_t>(kPoolAllowance,
+                                                        mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift)))));
+  const size_t B = z.B;
+  z.q = B * z.d * 4;
+  z.qcodes = B * z.dc;
+  z.qnorm = B * 8;
+  z.q4 = B * 4;
+  z.dist = B * z.nlist * 4;
+  z.probes = B * z.np * 4;
+  z.gcnt = z.nslots * 4ull;
+  z.pscratch = katoann_hip::plan_scratch_words(z.B, z.nlist, z.np, z.shift) * 4;
+  z.item_off = (z.nslots + 1ull) * 4;
+  z.pool_cnt = B * 4 * katoann_hip::kCntStride;
+  z.thr = B * 8;
+  z.state = z.nstate * 4ull;
+  z.items = static_cast<size_t>(z.item_cap) * 16;
+  z.out = B * z.K * 4;
+  z.surv = (x && x->survivors) ? B * z.S * 8 : 0;
+  z.codes = (x && x->codes) ? B * z.dc * 2 : 0;
+  const katoann_nprobe_model *model = x ? x->nprobe_model : nullptr;
+  if (model) {
+    if (x->nprobe_per_query) throw std::invalid_argument("nprobe_model and nprobe_per_query are exclusive");
+    if (model->nfeat == 0 || model->nfeat > z.np || model->hidden == 0)
+      throw std::invalid_argument("nprobe model needs 1 <= nfeat <= nprobe and hidden >= 1");
+    z.mx16 = B * model->nfeat * 2;
+    z.mh = B * model->hidden * 2;
+  }
+  z.npq = (model || (x && x->nprobe_per_query)) ? B * 4 : 0;
+  z.pool = std::max<uint64_t>(static_cast<uint64_t>(std::min<uint64_t>(z.S + z.allowance, 0xffffffffu)) * B, 1) * 8;
+  return z;
+}
 
-  const uint32_t B = nq, d = ix.dim, dc = ix.dc, np = p.nprobe, K = p.k, nlist = ix.nlist;
+void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
+                 const katoann_search_extras *x, uint32_t *out_ids, float *out_dists) {
+  if (nq == 0) return;
+  const SearchPlan z = plan_search(ix, nq, p, x);
+  const uint32_t B = z.B, d = z.d, dc = z.dc, np = z.np, K = z.K, nlist = z.nlist, S = z.S, P = z.P;
+  const bool prefix = z.prefix;
   const bool dev_io = x && x->device_io;
   auto &mix = const_cast<katoann_index &>(ix);
   auto ctx = ix.acquire();
@@ -431,64 +502,48 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   hipStream_t s = (dev_io && x->stream) ? static_cast<hipStream_t>(x->stream) : c.stream;
   const bool timing = ix.timing.load();
 
-  c.q.ensure(static_cast<size_t>(B) * d * 4);
-  c.y.ensure(static_cast<size_t>(B) * d * 4);
-  c.qhi.ensure(static_cast<size_t>(B) * dc);
-  c.qlo.ensure(static_cast<size_t>(B) * dc);
-  c.qnorm.ensure(B * 8ull);
-  c.qhn.ensure(B * 4ull);
-  c.qh0.ensure(B * 4ull);
-  c.dist.ensure(static_cast<size_t>(B) * nlist * 4);
-  c.probes.ensure(static_cast<size_t>(B) * np * 4);
-  c.probe_d.ensure(static_cast<size_t>(B) * np * 4);
-  const uint32_t shift = ix.band_shift;
-  const uint32_t qc = katoann_hip::rank_queries_per_item(P);
-  if (qc == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
-  const uint32_t nslots = katoann_hip::plan_slots(nlist, np, shift);
-  c.gcnt.ensure(nslots * 4ull);
-  c.qoff.ensure(nslots * 4ull);
-  c.pscratch.ensure(katoann_hip::plan_scratch_words(B, nlist, np, shift) * 4);
-  c.item_off.ensure((nslots + 1ull) * 4);
-  c.qlist.ensure(static_cast<size_t>(B) * np * 4);
+  c.q.ensure(z.q);
+  c.y.ensure(z.q);
+  c.qhi.ensure(z.qcodes);
+  c.qlo.ensure(z.qcodes);
+  c.qnorm.ensure(z.qnorm);
+  c.qhn.ensure(z.q4);
+  c.qh0.ensure(z.q4);
+  c.dist.ensure(z.dist);
+  c.probes.ensure(z.probes);
+  c.probe_d.ensure(z.probes);
+  const uint32_t shift = z.shift, qc = z.qc, nslots = z.nslots, nbands = z.nbands, nstate = z.nstate, rt = z.rt;
+  c.gcnt.ensure(z.gcnt);
+  c.qoff.ensure(z.gcnt);
+  c.pscratch.ensure(z.pscratch);
+  c.item_off.ensure(z.item_off);
+  c.qlist.ensure(z.probes);
   c.totals.ensure(16);
-  c.pool_cnt.ensure(B * 4ull * katoann_hip::kCntStride);
-  c.pool_sorted.ensure(B * 4ull);
-  c.thr.ensure(B * 8ull);
-  c.lbound.ensure(B * 4ull);
-  const uint32_t nbands = katoann_hip::plan_bands(np, shift);
-  if (nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
-  const uint32_t nstate = 3 + nbands;
-  c.state.ensure(nstate * 4ull);
-  const uint32_t rt = ix.rank_rows;
-  const uint64_t max_rows = mix.max_rows_per_query(np);
-  // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
-  // groups over all bands hold at most B queries
-  uint64_t tiles_total = 0;
-  for (uint32_t cc = 0; cc < nlist; ++cc) tiles_total += (ix.sizes[cc] + rt - 1) / rt;
-  const uint64_t item_cap64 = tiles_total * ((B + qc - 1) / qc + nbands);
-  if (item_cap64 > 0x7fffffffull) throw std::invalid_argument("batch too large for the ranking plan");
-  const uint32_t item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
-  c.items.ensure(static_cast<size_t>(item_cap) * 16);
-  c.ids.ensure(static_cast<size_t>(B) * K * 4);
-  c.dists.ensure(static_cast<size_t>(B) * K * 4);
-  if (x && x->survivors) c.surv.ensure(static_cast<size_t>(B) * S * 8);
-  if (x && x->n_survivors) c.nsurv.ensure(B * 4ull);
-  if (x && x->codes) c.codes.ensure(static_cast<size_t>(B) * dc * 2);
+  c.pool_cnt.ensure(z.pool_cnt);
+  c.pool_sorted.ensure(z.q4);
+  c.thr.ensure(z.thr);
+  c.lbound.ensure(z.q4);
+  c.state.ensure(z.state);
+  const uint64_t max_rows = z.max_rows;
+  const uint32_t item_cap = z.item_cap;
+  c.items.ensure(z.items);
+  c.ids.ensure(z.out);
+  c.dists.ensure(z.out);
+  if (z.surv) c.surv.ensure(z.surv);
+  if (x && x->n_survivors) c.nsurv.ensure(z.q4);
+  if (z.codes) c.codes.ensure(z.codes);
   const katoann_nprobe_model *model = x ? x->nprobe_model : nullptr;
   const bool coarse_only = x && x->coarse_only;
   if (model) {
-    if (x->nprobe_per_query) throw std::invalid_argument("nprobe_model and nprobe_per_query are exclusive");
-    if (model->nfeat == 0 || model->nfeat > np || model->hidden == 0)
-      throw std::invalid_argument("nprobe model needs 1 <= nfeat <= nprobe and hidden >= 1");
-    c.npq.ensure(B * 4ull);
-    c.mx16.ensure(static_cast<size_t>(B) * model->nfeat * 2);
-    c.mh1.ensure(static_cast<size_t>(B) * model->hidden * 2);
-    c.mh2.ensure(static_cast<size_t>(B) * model->hidden * 2);
-    c.mz.ensure(B * 4ull);
+    c.npq.ensure(z.npq);
+    c.mx16.ensure(z.mx16);
+    c.mh1.ensure(z.mh);
+    c.mh2.ensure(z.mh);
+    c.mz.ensure(z.q4);
   }
   const uint32_t *npq = nullptr;
   if (x && x->nprobe_per_query) {
-    c.npq.ensure(B * 4ull);
+    c.npq.ensure(z.npq);
     HIPCHECK(hipMemcpyAsync(c.npq.p, x->nprobe_per_query, B * 4ull,
                             dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     npq = c.npq.as<uint32_t>();
@@ -589,10 +644,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     HIPCHECK(hipStreamSynchronize(s));
     return st[0] == 0;
   };
-  // the first bands run without a threshold (or with one from few candidates):
-  // room for every row of a query's 1 + F nearest lists
-  const uint64_t allowance =
-      std::min(max_rows, std::max<uint64_t>(kPoolAllowance, mix.max_rows_per_query(std::min(np, 1u + (1u << shift)))));
+  const uint64_t allowance = z.allowance;
   if (!run(allowance) && (allowance >= max_rows || !run(max_rows))) {
     ix.release(std::move(ctx));
     throw std::runtime_error("survivor pool overflow in the ranking pass");
@@ -764,6 +816,14 @@ int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq,
       if (x->nprobe_per_query[q] == 0 || x->nprobe_per_query[q] > p->nprobe)
         throw std::invalid_argument("per-query nprobe must be in [1, nprobe]");
   search_impl(*h, queries, nq, *p, x, ids, dists);
+  KATOANN_CATCH
+}
+
+int katoann_search_footprint(const katoann_index *h, uint32_t nq, const katoann_search_params *p,
+                             const katoann_search_extras *x, uint64_t *bytes) {
+  KATOANN_TRY
+  if (!h || !p || !bytes) throw std::invalid_argument("null argument");
+  *bytes = nq ? plan_search(*h, nq, *p, x).total() : 0;
   KATOANN_CATCH
 }
 

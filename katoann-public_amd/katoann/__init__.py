@@ -61,6 +61,18 @@ class _Extras(C.Structure):
                 ("coarse_only", C.c_int32)]
 
 
+class ServerOptions(C.Structure):
+    """katoann_server_options: batches in flight, queries per batch, HBM budget (0: 90% of free)."""
+    _fields_ = [("max_inflight", C.c_uint32), ("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("queries", C.c_uint64), ("batches", C.c_uint64),
+                ("p50_ms", C.c_double), ("p95_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_batch", C.c_double), ("achieved_qps", C.c_double), ("budget", C.c_uint64),
+                ("peak_reserved", C.c_uint64), ("peak_inflight", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -115,6 +127,17 @@ def lib():
         L.katoann_mlp_pinball.argtypes = [P, P, u32, f32, P, P, P]
         L.katoann_mlp_adam.argtypes = [P, P, P, P, u32, f32, f32, f32, f32, u32, P, P]
         L.katoann_mlp_features.argtypes = [P, u32, u32, u32, P, P, P, P, P]
+        L.katoann_search_footprint.argtypes = [P, u32, C.POINTER(_SearchParams), C.POINTER(_Extras),
+                                               C.POINTER(u64)]
+        L.katoann_default_server_options.argtypes = [C.POINTER(ServerOptions)]
+        L.katoann_server_create.argtypes = [P, C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]
+        L.katoann_server_destroy.argtypes = [P]
+        L.katoann_server_submit.argtypes = [P, P, u32, C.POINTER(_SearchParams), C.POINTER(NprobeModel), i32,
+                                            C.POINTER(u64)]
+        L.katoann_server_poll.argtypes = [P, u64, C.POINTER(i32)]
+        L.katoann_server_wait.argtypes = [P, u64, P, P, C.POINTER(f64)]
+        L.katoann_server_drain.argtypes = [P]
+        L.katoann_server_stats_get.argtypes = [P, C.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -383,3 +406,76 @@ def rotate_rows(x, rotation):
     out = np.empty_like(x)
     _check(lib().katoann_rotate_rows(_ptr(x), _ptr(out), x.shape[0], x.shape[1], _ptr(r)))
     return out
+
+
+class Server:
+    """Continuous-batching request loop over a loaded Index (no reference
+    counterpart; replaces the CLI's synchronous `load` batching,
+    tools/katoann.cpp:267-337). submit() queues host queries and returns a
+    ticket at once; wait(ticket) returns (ids, dists, latency_ms)."""
+
+    def __init__(self, index: Index, max_inflight: int = 4, max_batch: int = 4096, hbm_budget: int = 0):
+        o = ServerOptions(max_inflight, max_batch, hbm_budget)
+        h = C.c_void_p()
+        _check(lib().katoann_server_create(index._h, C.byref(o), C.byref(h)))
+        self._h, self._index, self._k = h, index, {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.katoann_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, queries, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold",
+               model: NprobeModel | None = None, normalize=None) -> int:
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim != 2 or q.shape[1] != self._index.dim:
+            raise ValueError("queries must be (n, dim) float32")
+        p = self._index._params(k, nprobe, ranking, survivors, selection)
+        t = C.c_uint64()
+        norm = int(self._index.normalized if normalize is None else normalize)
+        _check(lib().katoann_server_submit(self._h, _ptr(q), q.shape[0], C.byref(p),
+                                           C.byref(model) if model is not None else None, norm, C.byref(t)))
+        self._k[t.value] = (q.shape[0], k)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int32()
+        _check(lib().katoann_server_poll(self._h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int):
+        nq, k = self._k.pop(ticket)
+        ids = np.empty((nq, k), np.uint32)
+        dists = np.empty((nq, k), np.float32)
+        lat = C.c_double()
+        _check(lib().katoann_server_wait(self._h, ticket, _ptr(ids), _ptr(dists), C.byref(lat)))
+        return ids, dists, lat.value
+
+    def drain(self):
+        _check(lib().katoann_server_drain(self._h))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().katoann_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def footprint(self, nq, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold",
+                  model: NprobeModel | None = None) -> int:
+        """Device bytes one search of nq queries with these parameters allocates."""
+        p = self._index._params(k, nprobe, ranking, survivors, selection)
+        x = _Extras()
+        if model is not None:
+            x.nprobe_model = C.pointer(model)
+        b = C.c_uint64()
+        _check(lib().katoann_search_footprint(self._index._h, nq, C.byref(p), C.byref(x), C.byref(b)))
+        return b.value

@@ -93,3 +93,8 @@ def test_search_argument_checks():
         ix.search(np.zeros((1, 0), np.float32))
     assert ix.engine == "gfx950"
     assert katoann.amx_available() is False
+
+
+def test_server_needs_a_loaded_index():
+    with pytest.raises(RuntimeError, match="not loaded"):
+        katoann.Server(katoann.Index())
