@@ -174,6 +174,12 @@ int katoann_set_rank_rows(katoann_index *h, uint32_t rows);
  * (F^(b+1) - 1) / (F - 1)) with F = 2^log2_factor (1..4); the queries' thresholds
  * tighten between bands. Default 2 (F = 4). */
 int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
+/* Ranking kernel: 0 = block items (a list's queries staged into LDS, rows
+ * streamed), 1 = wave items (a wave keeps up to 16 T rows of a list in
+ * registers and streams up to queries_per_item of its queries), 2 = block items
+ * of 4 x 16 T rows whose 4 waves share the queries through an LDS ring;
+ * variant 0 or 1 picks the larger or smaller T. Default: 1, variant 0, 512. */
+int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item);
 
 /* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
  * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of

@@ -114,6 +114,7 @@ struct katoann_index {
   uint64_t nrows = 0;
   uint32_t rank_rows = 1024;
   uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
+  uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 512;
   std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
   uint32_t tile_prefix_rows = 0;
 
@@ -404,8 +405,8 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // Validated shape of one search and the bytes of every per-search buffer, so
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
-  uint32_t B, d, dc, np, K, S, nlist, P, shift, qc, nslots, nbands, nstate, rt, item_cap;
-  bool prefix;
+  uint32_t B, d, dc, np, K, S, nlist, P, shift, qc, nslots, nbands, nstate, rt, item_cap, variant;
+  bool prefix, wave, ring;
   uint64_t max_rows, allowance;
   size_t q, qcodes, qnorm, q4, dist, probes, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
       codes, npq, mx16, mh, pool;
@@ -439,13 +440,16 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   if (z.P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
   z.B = nq, z.d = ix.dim, z.dc = ix.dc, z.np = p.nprobe, z.K = p.k, z.nlist = ix.nlist;
   z.shift = ix.band_shift;
-  z.qc = katoann_hip::rank_queries_per_item(z.P);
-  if (z.qc == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+  z.wave = ix.rank_kernel >= 1;
+  z.ring = ix.rank_kernel == 2;
+  z.variant = ix.wave_variant;
+  z.qc = z.wave ? ix.wave_qc : katoann_hip::rank_queries_per_item(z.P);
+  z.rt = z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1) : ix.rank_rows;
+  if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
   z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift);
   z.nbands = katoann_hip::plan_bands(z.np, z.shift);
   if (z.nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
   z.nstate = 3 + z.nbands;
-  z.rt = ix.rank_rows;
   auto &mix = const_cast<katoann_index &>(ix);
   z.max_rows = mix.max_rows_per_query(z.np);
   // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
@@ -618,12 +622,18 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
     for (uint32_t band = 0; band < nbands; ++band) {
       if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
-      if (!katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, c.qhi.as<int8_t>(),
-                                         dc, qn, c.lbound.as<int32_t>(), c.qlist.as<uint32_t>(),
-                                         c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride, ix.step_sq,
-                                         c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(),
-                                         grid, s))
-        throw std::invalid_argument("unsupported ranking shape (prefix dims)");
+      const bool launched =
+          z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
+                                                 c.qhi.as<int8_t>(), dc, qn, c.lbound.as<int32_t>(),
+                                                 c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
+                                                 c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
+                                                 c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s)
+                 : katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
+                                                 c.qhi.as<int8_t>(), dc, qn, c.lbound.as<int32_t>(),
+                                                 c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
+                                                 c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
+                                                 c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
+      if (!launched) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
       if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band + 1], s));
       if (!katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
                                       c.thr.as<uint64_t>(), c.lbound.as<int32_t>(), B, S, stride, P, ix.step_sq, s))
@@ -916,6 +926,18 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
   if (log2_factor < 1 || log2_factor > 4) throw std::invalid_argument("band factor must be 2, 4, 8 or 16");
   std::lock_guard<std::mutex> g(h->mu);
   h->band_shift = log2_factor;
+  KATOANN_CATCH
+}
+
+int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item) {
+  KATOANN_TRY
+  if (kernel > 2 || variant > 1) throw std::invalid_argument("rank kernel is 0, 1 or 2, variant 0 or 1");
+  if (queries_per_item == 0 || queries_per_item > 65536)
+    throw std::invalid_argument("queries per ranking item must be in 1..65536");
+  std::lock_guard<std::mutex> g(h->mu);
+  h->rank_kernel = kernel;
+  h->wave_variant = variant;
+  h->wave_qc = queries_per_item;
   KATOANN_CATCH
 }
 

@@ -37,53 +37,91 @@ __global__ void k_normalize(float *x, uint32_t n, uint32_t d) {
 // addition is commutative, so every lane of the butterfly ends with the same
 // bits); then FMAs over k >= d & ~15 in order.
 //
-// Tile: 32 rows x 32 basis rows per 256-thread block. Thread = (chain l, slot);
-// the 4 slots of a wave share 8 input rows (LDS broadcast) and own 8 basis
-// rows each, so each thread keeps 8 x 8 chain accumulators.
-constexpr int RT_TILE = 32;
+// Tile: 64 rows x 32 basis rows per 512-thread block. Thread = (chain l,
+// slot); the 4 slots of a wave share 8 input rows (LDS broadcast) and own 8
+// basis rows each, so each thread keeps 8 x 8 chain accumulators. The next
+// 64-wide chunk of k is loaded into registers while the current one is
+// multiplied out of LDS (double buffer, one barrier per chunk).
+constexpr int RT_ROWS = 64;
+constexpr int RT_COLS = 32;
 constexpr int RT_K = 64;
 constexpr int RT_LD = RT_K + 2;  // 66 floats: the four slots of a wave hit disjoint banks
 
-__global__ __launch_bounds__(256) void k_rotate(const float *__restrict__ X, uint32_t n, uint32_t d,
+__global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uint32_t n, uint32_t d,
                                                 const float *__restrict__ R, float *__restrict__ Y) {
-  __shared__ float xs[RT_TILE][RT_LD];
-  __shared__ float rs[RT_TILE][RT_LD];
+  __shared__ float xs[2][RT_ROWS][RT_LD];
+  __shared__ float rs[2][RT_COLS][RT_LD];
   const int tid = threadIdx.x;
   const int l = tid & 15;
   const int slot = tid >> 4;
   const int rg = slot >> 2, cg = slot & 3;
-  const uint32_t i0 = blockIdx.x * RT_TILE, j0 = blockIdx.y * RT_TILE;  // rows on x: no 65535 limit
+  const uint32_t i0 = blockIdx.x * RT_ROWS, j0 = blockIdx.y * RT_COLS;  // rows on x: no 65535 limit
   const uint32_t d16 = d & ~15u, d4 = d & ~3u;
+  // loader: thread t moves 4 consecutive k of input rows t / 16 and 32 + t / 16
+  // and of basis row t / 16 per chunk
+  const int lrow = tid >> 4, lk = 4 * (tid & 15);
+  const float *xrow0 = X + static_cast<size_t>(min(i0 + lrow, n - 1)) * d;
+  const float *xrow1 = X + static_cast<size_t>(min(i0 + 32 + lrow, n - 1)) * d;
+  const float *rrow = R + static_cast<size_t>(min(j0 + lrow, d - 1)) * d;
+  const bool vec = (d & 3u) == 0;
+  float4 p0, p1, pr;
+  auto ld4 = [&](const float *r, uint32_t k) {
+    return vec ? *reinterpret_cast<const float4 *>(r + k) : float4{r[k], r[k + 1], r[k + 2], r[k + 3]};
+  };
+  auto fetch = [&](uint32_t k0) {
+    const uint32_t k = k0 + lk;  // k < d16 for all four or for none (d16 % 16 == 0)
+    if (k >= d16) {
+      p0 = p1 = pr = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    } else {
+      p0 = ld4(xrow0, k);
+      p1 = ld4(xrow1, k);
+      pr = ld4(rrow, k);
+    }
+  };
+  auto put4 = [&](float *dst, float4 v) {
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+    dst[3] = v.w;
+  };
+  auto stash = [&](int b) {
+    put4(&xs[b][lrow][lk], p0);
+    put4(&xs[b][32 + lrow][lk], p1);
+    put4(&rs[b][lrow][lk], pr);
+  };
   float acc[8][8];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = 0.0f;
 
+  if (d16) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
   for (uint32_t k0 = 0; k0 < d16; k0 += RT_K) {
-    __syncthreads();
-    for (int e = tid; e < RT_TILE * RT_K; e += 256) {
-      const int rr = e / RT_K, kk = e % RT_K;
-      const uint32_t k = k0 + kk;
-      const uint32_t xi = min(i0 + rr, n - 1), rj = min(j0 + rr, d - 1);
-      xs[rr][kk] = k < d16 ? X[static_cast<size_t>(xi) * d + k] : 0.0f;
-      rs[rr][kk] = k < d16 ? R[static_cast<size_t>(rj) * d + k] : 0.0f;
-    }
-    __syncthreads();
+    const int buf = (k0 / RT_K) & 1;
+    const bool more = k0 + RT_K < d16;
+    if (more) fetch(k0 + RT_K);
 #pragma unroll
     for (int m = 0; m < RT_K / 16; ++m) {
       if (k0 + 16 * m < d16) {  // uniform: d16 is a multiple of 16
         const int kk = 16 * m + l;
         float xv[8], rv[8];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) xv[a] = xs[rg * 8 + a][kk];
+        for (int a = 0; a < 8; ++a) xv[a] = xs[buf][rg * 8 + a][kk];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) rv[b] = rs[cg * 8 + b][kk];
+        for (int b = 0; b < 8; ++b) rv[b] = rs[buf][cg * 8 + b][kk];
 #pragma unroll
         for (int a = 0; a < 8; ++a)
 #pragma unroll
           for (int b = 0; b < 8; ++b) acc[a][b] = fmaf(xv[a], rv[b], acc[a][b]);
       }
+    }
+    if (more) {
+      stash(buf ^ 1);
+      __syncthreads();
     }
   }
   // butterfly = reduce tree; every one of the 16 lanes ends with the sum
@@ -234,49 +272,103 @@ __device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_coarse(const int8_t *__restrict__ qhi, const int8_t *__restrict__ qlo,
-                                                const int64_t *__restrict__ qnorm, uint32_t B,
-                                                const int8_t *__restrict__ chi, const int8_t *__restrict__ clo,
-                                                const int64_t *__restrict__ cnorm, uint32_t nlist, uint32_t dc,
-                                                double step_sq, float *__restrict__ dist) {
-  const int lane = lane_id(), w = threadIdx.x >> 6;
-  const uint32_t q0 = blockIdx.y * 64 + 16 * w;
-  const uint32_t c0 = blockIdx.x * 64;
-  if (q0 >= B) return;
-  const uint32_t qrow = min(q0 + (lane & 15), B - 1);
-  const uint32_t koff = 16 * (lane >> 4);
-  uint32_t crow[4];
+// Block tile: 128 queries x 128 centroids, 4 waves, wave (wq, wc) owns the
+// 64 x 64 quadrant = 4 x 4 MFMA tiles with three accumulators each (hh, hl + lh,
+// ll). Per 64-byte chunk of the codes the four operand planes (queries and
+// centroids, high and low) are staged in LDS in MFMA fragment order (lane l's
+// 16 bytes at l * 16 of a 1 KB tile fragment, so every read is conflict-free);
+// the next chunk is loaded into registers while the current one is multiplied.
+constexpr int CO_T = 128;                    // queries and centroids per block tile
+constexpr int CO_STAGE = 4 * CO_T * 64;      // bytes of one chunk: 4 planes x 128 rows x 64
+
+__global__ __launch_bounds__(256, 1) void k_coarse(const int8_t *__restrict__ qhi, const int8_t *__restrict__ qlo,
+                                                   const int64_t *__restrict__ qnorm, uint32_t B,
+                                                   const int8_t *__restrict__ chi, const int8_t *__restrict__ clo,
+                                                   const int64_t *__restrict__ cnorm, uint32_t nlist, uint32_t dc,
+                                                   double step_sq, float *__restrict__ dist) {
+  __shared__ __attribute__((aligned(16))) int8_t st[2][CO_STAGE];
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const int wq = w >> 1, wc = w & 1;
+  const uint32_t q0 = blockIdx.y * CO_T, c0 = blockIdx.x * CO_T;
+  // loader: thread t moves (row t / 4 and 64 + t / 4, slice t % 4) of each plane;
+  // plane order in a stage: query high, query low, centroid high, centroid low
+  const int lr = tid >> 2, ls = tid & 3;
+  const int8_t *src[8];
+  uint32_t dst[8];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) crow[t] = min(c0 + 16 * t + (lane & 15), nlist - 1);
-  v4i hh[4], mid[4], ll[4];
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t r = static_cast<uint32_t>(lr + 64 * h);
+    const uint32_t qr = min(q0 + r, B - 1), cr = min(c0 + r, nlist - 1);
+    src[0 + h] = qhi + static_cast<size_t>(qr) * dc + 16 * ls;
+    src[2 + h] = qlo + static_cast<size_t>(qr) * dc + 16 * ls;
+    src[4 + h] = chi + static_cast<size_t>(cr) * dc + 16 * ls;
+    src[6 + h] = clo + static_cast<size_t>(cr) * dc + 16 * ls;
+    // row r of a plane: tile r / 16, lane (slice, r % 16) of that tile's fragment
+    const uint32_t off = (r >> 4) * 1024 + (ls * 16 + (r & 15)) * 16;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) hh[t] = mid[t] = ll[t] = v4i{0, 0, 0, 0};
-  for (uint32_t kc = 0; kc < dc; kc += 64) {
-    const v4i ah = load16(qhi + static_cast<size_t>(qrow) * dc + kc + koff);
-    const v4i al = load16(qlo + static_cast<size_t>(qrow) * dc + kc + koff);
+    for (int pl = 0; pl < 4; ++pl) dst[2 * pl + h] = pl * (CO_T * 64) + off;
+  }
+  v4i pre[8];
+  auto fetch = [&](uint32_t kc) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pre[i] = load16(src[i] + kc);
+  };
+  auto stash = [&](int b) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) *reinterpret_cast<v4i *>(&st[b][dst[i]]) = pre[i];
+  };
+  v4i hh[4][4], mid[4][4], ll[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
+  const uint32_t nk = dc / 64;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (uint32_t k = 0; k < nk; ++k) {
+    const int b = k & 1;
+    if (k + 1 < nk) fetch((k + 1) * 64);
+    const int8_t *S = st[b];
+    v4i bh[4], bl[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const v4i bh = load16(chi + static_cast<size_t>(crow[t]) * dc + kc + koff);
-      const v4i bl = load16(clo + static_cast<size_t>(crow[t]) * dc + kc + koff);
-      hh[t] = mfma(ah, bh, hh[t]);
-      mid[t] = mfma(ah, bl, mid[t]);
-      mid[t] = mfma(al, bh, mid[t]);
-      ll[t] = mfma(al, bl, ll[t]);
+      bh[t] = *reinterpret_cast<const v4i *>(S + 2 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
+      bl[t] = *reinterpret_cast<const v4i *>(S + 3 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const v4i ah = *reinterpret_cast<const v4i *>(S + (4 * wq + a) * 1024 + lane * 16);
+      const v4i al = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + a) * 1024 + lane * 16);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        hh[a][t] = mfma(ah, bh[t], hh[a][t]);
+        mid[a][t] = mfma(ah, bl[t], mid[a][t]);
+        mid[a][t] = mfma(al, bh[t], mid[a][t]);
+        ll[a][t] = mfma(al, bl[t], ll[a][t]);
+      }
+    }
+    if (k + 1 < nk) {
+      stash(b ^ 1);
+      __syncthreads();
     }
   }
+  // C[i][j] of tile (a, t) sits in lane 16 (i / 4) + j, element i % 4
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const uint32_t c = c0 + 16 * t + (lane & 15);
+    const uint32_t c = c0 + 64 * wc + 16 * t + (lane & 15);
     if (c >= nlist) continue;
     const int64_t cn = cnorm[c];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const uint32_t q = q0 + 4 * (lane >> 4) + r;
-      if (q >= B) continue;
-      const int64_t dot = static_cast<int64_t>(hh[t][r]) * 65536 + static_cast<int64_t>(mid[t][r]) * 256 +
-                          static_cast<int64_t>(ll[t][r]);
-      dist[static_cast<size_t>(q) * nlist + c] = fscale(qnorm[q] + cn - 2 * dot, step_sq);
-    }
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t q = q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r;
+        if (q >= B) continue;
+        const int64_t dot = static_cast<int64_t>(hh[a][t][r]) * 65536 + static_cast<int64_t>(mid[a][t][r]) * 256 +
+                            static_cast<int64_t>(ll[a][t][r]);
+        dist[static_cast<size_t>(q) * nlist + c] = fscale(qnorm[q] + cn - 2 * dot, step_sq);
+      }
   }
 }
 
@@ -441,19 +533,18 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
 // when its farther lists are ranked. Pair (q, c) is probed iff the key
 // (coarse distance bits, c) is at most the key of q's last probe, and its band
 // is the number of band-start keys of q at or below it; the coarse matrix is
-// read in tiles of 64 lists x 256 queries:
-//   k_inv_count  per-tile counts of every (band, list) slot (LDS),
-//   k_slot_totals per slot: the tiles' offsets within it, its query count and
-//                its items; per 1024 slots: their sums,
-//   k_slot_scan  slot and item offsets (exclusive scans),
-//   k_inv_fill   the query lists of every slot.
+// read in tiles of 256 lists x 256 queries:
+//   k_inv_pass<false> per-tile counts of every (band, list) slot,
+//   k_slot_totals     per slot: the tiles' offsets within it, its query count
+//                     and its items; per 1024 slots: their sums,
+//   k_slot_scan       slot and item offsets (exclusive scans),
+//   k_inv_pass<true>  the query lists of every slot (ascending queries).
 __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-constexpr int INV_LT = 64;    // lists per tile
-constexpr int INV_QT = 256;   // queries per tile, one per thread
+constexpr int INV_QT = 64;    // queries per tile
 constexpr int INV_MAXB = 16;  // bands at most (nprobe 4096, F = 2: 13)
 
 __device__ __forceinline__ uint64_t probe_key(const uint32_t *probes, const float *probe_d, size_t i) {
@@ -487,101 +578,64 @@ __device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_
   return r;
 }
 
-// Band of key k for the query, or -1 when the list is not probed.
-__device__ __forceinline__ int inv_band(const InvQuery &r, uint64_t k, uint32_t nb) {
-  if (!r.valid || k > r.kend) return -1;
-  int band = 0;
-#pragma unroll
-  for (int b = 1; b < INV_MAXB; ++b)
-    if (static_cast<uint32_t>(b) < nb) band += k >= r.kb[b] ? 1 : 0;
-  return band;
-}
+// One thread per list, walking a tile of INV_QT queries in ascending order
+// (reads of the coarse matrix coalesced across the lists): counts of each
+// (band, list) slot in the tile (FILL false), or the tile's queries appended to
+// the slots' query lists from the tile's offsets (FILL true). The queries'
+// band-start keys are staged in LDS and read as broadcasts.
+constexpr int INV_LB = 256;  // lists per block
 
-// Walks the tile's lists for the thread's query, calling f(list index in tile, band).
-template <class F>
-__device__ __forceinline__ void inv_walk(const float *__restrict__ dist, const InvQuery &r, uint32_t q, uint32_t B,
-                                         uint32_t nlist, uint32_t c0, uint32_t nb, F &&f) {
-  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
-  const float *row = dist + static_cast<size_t>(min(q, B - 1)) * nlist + c0;
-  const bool vec = (nlist % 4 == 0) && nt == INV_LT;
-  for (uint32_t i0 = 0; i0 < nt; i0 += 16) {
-    float v[16];
-    if (vec) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float4 x = *reinterpret_cast<const float4 *>(row + i0 + 4 * j);
-        v[4 * j] = x.x;
-        v[4 * j + 1] = x.y;
-        v[4 * j + 2] = x.z;
-        v[4 * j + 3] = x.w;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 16; ++j) v[j] = i0 + j < nt ? row[i0 + j] : 0.0f;
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      if (i0 + j >= nt) break;
-      const uint64_t k = (static_cast<uint64_t>(__float_as_uint(v[j])) << 32) | (c0 + i0 + j);
-      f(i0 + j, inv_band(r, k, nb));
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_inv_count(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
-                                                   const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
-                                                   uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb,
-                                                   uint32_t shift, uint32_t *part) {
-  __shared__ uint32_t cnt[INV_LT * INV_MAXB];
-  const uint32_t c0 = blockIdx.x * INV_LT, qt = blockIdx.y, q = qt * INV_QT + threadIdx.x;
-  const uint32_t nslots = nb * nlist;
-  const int lane = lane_id();
-  for (int i = threadIdx.x; i < INV_LT * INV_MAXB; i += 256) cnt[i] = 0;
-  __syncthreads();
-  const InvQuery r = inv_query(probes, probe_d, npq, q, B, np, nb, shift);
-  inv_walk(dist, r, q, B, nlist, c0, nb, [&](uint32_t i, int band) {
-    for (uint32_t b = 0; b < nb; ++b) {
-      const uint64_t m = ballot(band == static_cast<int>(b));
-      if (m && lane == 0) atomicAdd(&cnt[i * INV_MAXB + b], static_cast<uint32_t>(__popcll(m)));
-    }
-  });
-  __syncthreads();
-  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
-  for (uint32_t e = threadIdx.x; e < nt * nb; e += 256) {
-    const uint32_t i = e / nb, b = e % nb;
-    part[static_cast<size_t>(qt) * nslots + b * nlist + c0 + i] = cnt[i * INV_MAXB + b];
-  }
-}
-
-__global__ __launch_bounds__(256) void k_inv_fill(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
                                                   const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
-                                                  uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb, uint32_t shift,
-                                                  const uint32_t *__restrict__ part, const uint32_t *__restrict__ qoff,
-                                                  uint32_t *__restrict__ qlist) {
-  __shared__ uint32_t pos[INV_LT * INV_MAXB];
-  const uint32_t c0 = blockIdx.x * INV_LT, qt = blockIdx.y, q = qt * INV_QT + threadIdx.x;
+                                                  uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb,
+                                                  uint32_t shift, uint32_t *__restrict__ part,
+                                                  const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist) {
+  __shared__ uint64_t s_kend[INV_QT];
+  __shared__ uint64_t s_kb[INV_MAXB][INV_QT];
+  const uint32_t qt = blockIdx.y, q0 = qt * INV_QT, c = blockIdx.x * INV_LB + threadIdx.x;
   const uint32_t nslots = nb * nlist;
-  const int lane = lane_id();
-  const uint32_t nt = min(static_cast<uint32_t>(INV_LT), nlist - c0);
-  for (uint32_t e = threadIdx.x; e < nt * nb; e += 256) {
-    const uint32_t i = e / nb, b = e % nb, slot = b * nlist + c0 + i;
-    pos[i * INV_MAXB + b] = qoff[slot] + part[static_cast<size_t>(qt) * nslots + slot];
+  if (threadIdx.x < INV_QT) {
+    const InvQuery r = inv_query(probes, probe_d, npq, q0 + threadIdx.x, B, np, nb, shift);
+    s_kend[threadIdx.x] = r.kend;
+#pragma unroll
+    for (int b = 0; b < INV_MAXB; ++b) s_kb[b][threadIdx.x] = r.kb[b];
   }
   __syncthreads();
-  const InvQuery r = inv_query(probes, probe_d, npq, q, B, np, nb, shift);
-  const uint64_t lt = (1ull << lane) - 1ull;
-  inv_walk(dist, r, q, B, nlist, c0, nb, [&](uint32_t i, int band) {
-    for (uint32_t b = 0; b < nb; ++b) {
-      const bool mine = band == static_cast<int>(b);
-      const uint64_t m = ballot(mine);
-      if (!m) continue;
-      const int lead = __ffsll(static_cast<long long>(m)) - 1;
-      uint32_t base = 0;
-      if (lane == lead) base = atomicAdd(&pos[i * INV_MAXB + b], static_cast<uint32_t>(__popcll(m)));
-      base = __shfl(base, lead);
-      if (mine) qlist[base + __popcll(m & lt)] = q;
+  if (c >= nlist) return;
+  const uint32_t nq = min(static_cast<uint32_t>(INV_QT), B - q0);
+  uint32_t acc[INV_MAXB];
+#pragma unroll
+  for (int b = 0; b < INV_MAXB; ++b) {
+    const size_t slot = static_cast<size_t>(b) * nlist + c;
+    acc[b] = (FILL && static_cast<uint32_t>(b) < nb) ? qoff[slot] + part[static_cast<size_t>(qt) * nslots + slot] : 0u;
+  }
+  const float *col = dist + static_cast<size_t>(q0) * nlist + c;
+  constexpr int U = 8;
+  for (uint32_t i0 = 0; i0 < nq; i0 += U) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = i0 + u < nq ? __float_as_uint(col[static_cast<size_t>(i0 + u) * nlist]) : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u;
+      if (i >= nq) break;
+      const uint64_t k = (static_cast<uint64_t>(v[u]) << 32) | c;
+      if (k > s_kend[i]) continue;
+      uint32_t band = 0;
+#pragma unroll
+      for (int b = 1; b < INV_MAXB; ++b)
+        if (static_cast<uint32_t>(b) < nb) band += k >= s_kb[b][i] ? 1u : 0u;
+#pragma unroll
+      for (int b = 0; b < INV_MAXB; ++b)
+        if (band == static_cast<uint32_t>(b)) {
+          if (FILL) qlist[acc[b]] = q0 + i;
+          ++acc[b];
+        }
     }
-  });
+  }
+  if (!FILL)
+    for (uint32_t b = 0; b < nb; ++b) part[static_cast<size_t>(qt) * nslots + static_cast<size_t>(b) * nlist + c] = acc[b];
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -751,8 +805,8 @@ void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float
   if (!n) return;
   const uint32_t d4 = d & ~3u;
   if (d4) {
-    dim3 grid((n + RT_TILE - 1) / RT_TILE, (d4 + RT_TILE - 1) / RT_TILE);
-    hipLaunchKernelGGL(k_rotate, grid, dim3(256), 0, s, X, n, d, R, Y);
+    dim3 grid((n + RT_ROWS - 1) / RT_ROWS, (d4 + RT_COLS - 1) / RT_COLS);
+    hipLaunchKernelGGL(k_rotate, grid, dim3(512), 0, s, X, n, d, R, Y);
   }
   if (d4 < d) {
     const uint64_t t = static_cast<uint64_t>(n) * (d - d4);
@@ -776,7 +830,7 @@ void launch_minmax(const float *y, uint64_t n, float *part, uint32_t nblocks, hi
 void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, uint32_t B, const int8_t *chi,
                    const int8_t *clo, const int64_t *cnorm, uint32_t nlist, uint32_t dc, double step_sq,
                    float *dist, hipStream_t s) {
-  dim3 grid((nlist + 63) / 64, (B + 63) / 64);
+  dim3 grid((nlist + CO_T - 1) / CO_T, (B + CO_T - 1) / CO_T);
   hipLaunchKernelGGL(k_coarse, grid, dim3(256), 0, s, qhi, qlo, qnorm, B, chi, clo, cnorm, nlist, dc, step_sq,
                      dist);
 }
@@ -816,14 +870,15 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   const uint32_t nb = plan_bands(np, shift), nslots = nb * nlist;
   const uint32_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
-  const dim3 tiles((nlist + INV_LT - 1) / INV_LT, qtiles);
+  const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
-  hipLaunchKernelGGL(k_inv_count, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift, part);
+  hipLaunchKernelGGL(k_inv_pass<false>, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift,
+                     part, nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, gcnt,
                      nit, bsum);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
-  hipLaunchKernelGGL(k_inv_fill, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift, part,
-                     qoff, qlist);
+  hipLaunchKernelGGL(k_inv_pass<true>, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift,
+                     part, qoff, qlist);
   hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr,
                      lbound);
 }

@@ -50,6 +50,15 @@ bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, 
                       const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,
                       double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state, uint32_t grid,
                       hipStream_t s);
+// Wave-item ranking kernels (wave-independent, or a block of 4 waves sharing a
+// query ring when `ring`): rows per wave for ranking width P and register
+// variant 0 (large) or 1 (half); 0 if unsupported.
+uint32_t rank_wave_rows(uint32_t P, uint32_t variant);
+bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                      uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                      const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                      uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                      uint32_t grid, hipStream_t s);
 uint32_t refine_cap(uint32_t S);
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
                    uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s);

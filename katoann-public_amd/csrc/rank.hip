@@ -12,12 +12,11 @@
 //
 // 1. Probe ranks are split into bands whose width grows geometrically (the
 //    first band is each query's nearest list). For each band:
-//    k_rank_band (list-major, persistent) takes items (list, up to QMAX of the
-//    band's queries of that list, up to a row range) from a device counter. The
-//    item's query codes are staged once into LDS in MFMA fragment order; each
-//    wave streams chunks of the list's rows straight from HBM into registers as
-//    MFMA operands (the next chunk in flight while the current one is ranked),
-//    so every row is read once per item and the loads of all waves overlap.
+//    the ranking kernel (list-major, persistent) runs items (list, up to QC of
+//    the band's queries of that list, up to a row range), dealt round-robin:
+//    k_rank_wave (default) keeps an item's rows in a wave's registers and
+//    streams its queries through; k_rank_band stages an item's queries in LDS
+//    and streams its rows through the block's waves.
 //    A candidate whose integer l2 is within its query's bound is appended to
 //    the query's pool. k_update then keeps each pool's S smallest keys and
 //    lowers the threshold (and its integer bound) to the S-th: nearer lists
@@ -65,7 +64,7 @@ struct BandArgs {
   uint32_t stride;           // pool slots per query
   uint64_t *pool;            // B x stride
   uint32_t *pool_cnt;        // B, kCntStride apart
-  uint32_t *state;           // [0] overflow, [1] kept, [3 + band] queue head
+  uint32_t *state;           // [0] overflow, [1] kept
   double step_sq;
 };
 
@@ -335,10 +334,14 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
     nst = 0;
   };
 
+  // items dealt round-robin over the blocks (a shared queue counter
+  // serialises every taker on one address): block k takes k, k + G, ...
+  uint32_t next_item = ib + blockIdx.x;
   if (tid == 0) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const uint32_t x = ib + atomicAdd(&a.state[3 + a.band], 1u);
+      const uint32_t x = next_item;
+      next_item += gridDim.x;
       s_idx[i] = x;
       s_desc[i] = x < ie ? a.items[x] : uint4{0, 1, 0, 1};
     }
@@ -364,7 +367,10 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
     const ItemSplit sp = split_of<RC>(d), nsp = split_of<RC>(nd);
     uint32_t i2 = 0;
     uint4 d2 = {0, 1, 0, 1};
-    if (tid == 0) i2 = ib + atomicAdd(&a.state[3 + a.band], 1u);
+    if (tid == 0) {
+      i2 = next_item;
+      next_item += gridDim.x;
+    }
     if (nvalid) stage1(nd, k + 1);
 
     const uint32_t aA = lds_addr(bufA(k)), aN = lds_addr(bufN(k)), aL = lds_addr(bufL(k));
@@ -465,6 +471,335 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
+// --------------------------------------------------------------------------
+// Ranking pass of one probe-rank band, wave-independent form. Each wave takes
+// items (list, up to 16 T rows, up to QC of the band's queries of the list)
+// from the band's counter, keeps the item's rows in registers as MFMA B
+// operands, and streams the item's queries through 16 at a time as A operands
+// loaded straight from the query codes (a few MB, L2-resident): the query ids
+// two fragments ahead, codes, norms and bounds one fragment ahead. No LDS and
+// no block barriers: the waves of a CU overlap one another's row loads, MFMAs
+// and filters freely.
+//   Per row x: a = floor(xn / 2); per query: h = qn - L.
+//   Keep  <=>  qn + xn - 2 dot <= L  <=>  2 dot - xn >= h, which needs
+//   dot - a >= floor(h / 2): the hot test is one add per row, a max over the
+//   rows and one compare per query. Fragments that pass it take the exact
+//   test and form the key f(qn + xn - 2 dot).
+// Kept keys go straight to the pools: one returning atomic per query of the
+// fragment (16 lanes, one round trip) reserves its slots. Items are dealt to
+// the waves round-robin (no shared counter).
+struct WaveArgs {
+  const int8_t *plane;       // N x P, row-major, shifted high bytes
+  const int32_t *xnorm;      // N
+  const uint32_t *ids;       // N
+  const int8_t *qhi;         // B x dc
+  uint32_t dc;
+  const int32_t *qnorm;      // B
+  const int32_t *lbound;     // B
+  const uint32_t *qlist;
+  const uint32_t *item_off;  // per (band, list) slot, nslots + 1
+  const uint4 *items;        // first query slot, queries, first row, end row
+  uint32_t nlist;
+  uint32_t stride;
+  uint64_t *pool;
+  uint32_t *pool_cnt;
+  uint32_t *state;           // [0] overflow, [1] kept
+  uint32_t band;
+  double step_sq;
+};
+
+template <int KC, int T>
+__global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
+  constexpr uint32_t P = 64 * KC;
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  const uint64_t gm = 0xffffull << (16 * g16);
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  uint32_t n_kept = 0;
+  // items dealt round-robin over the grid's waves: a shared queue counter
+  // serialises every taker on one address
+  const uint32_t nw = gridDim.x * 4;
+  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw) {
+    const uint4 d = a.items[it];
+    // the item's rows: B operands, norms and ids
+    v4i b[T * KC];
+    int32_t xn[T];
+    uint32_t xid[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const uint32_t row = min(d.z + 16 * t + c16, d.w - 1);
+      const int8_t *rp = a.plane + static_cast<size_t>(row) * P + 16 * g16;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) b[t * KC + kc] = load16(rp + 64 * kc);
+      xn[t] = a.xnorm[row];
+      xid[t] = a.ids[row];
+    }
+    const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
+    const uint32_t *ql = a.qlist + d.x;
+    // query of lane c16 in fragment f (clamped into the item)
+    auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
+    uint32_t qid1 = qid_of(0), qid2 = nf > 1 ? qid_of(1) : qid1;
+    v4i An[KC];
+    int32_t qnn, hn;
+    auto load_q = [&](uint32_t q, v4i(&A)[KC], int32_t &qn, int32_t &h) {
+      const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) A[kc] = load16(src + 64 * kc);
+      qn = a.qnorm[q];
+      h = qn - a.lbound[q];
+    };
+    load_q(qid1, An, qnn, hn);
+    for (uint32_t f = 0; f < nf; ++f) {
+      v4i A[KC];
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) A[kc] = An[kc];
+      const int32_t qn = qnn, h = hn;
+      const uint32_t qid = qid1;
+      qid1 = qid2;
+      if (f + 1 < nf) load_q(qid1, An, qnn, hn);
+      if (f + 2 < nf) qid2 = qid_of(f + 2);
+      v4i acc[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
+      // h of the lane's 4 queries 4 g16 + r (no query past the item)
+      int32_t h4[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t j = 4 * g16 + r;
+        const int32_t hv = __shfl(h, static_cast<int>(j));
+        h4[r] = 16 * f + j < nq ? hv : INT32_MAX;
+      }
+      bool any = false;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int32_t m = acc[0][r] - (xn[0] >> 1);
+#pragma unroll
+        for (int t = 1; t < T; ++t) m = max(m, acc[t][r] - (xn[t] >> 1));
+        any |= m >= (h4[r] >> 1);
+      }
+      if (!ballot(any)) continue;
+      // exact test, per-query counts in the lane's group, one reservation per query
+      auto keep = [&](int t, int r) {
+        return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r];
+      };
+      uint32_t cnt[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cnt[r] = 0;
+#pragma unroll
+        for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
+      }
+      const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
+      const uint32_t qmine = __shfl(qid, 4 * g16 + (c16 & 3));
+      uint32_t base = 0;
+      if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
+      // the tests again rather than 4 T masks kept live from the count
+#pragma unroll
+      for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int src = 4 * g16 + r;
+        uint32_t off = __shfl(base, 16 * g16 + r);
+        const uint32_t q = __shfl(qid, src);
+        const int32_t qnr = __shfl(qn, src);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const bool bit = keep(t, r);
+          const uint64_t m = ballot(bit) & gm;
+          if (bit) {
+            const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+            const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
+            if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+            else atomicOr(&a.state[0], 1u);
+          }
+          off += static_cast<uint32_t>(__popcll(m));
+        }
+      }
+      n_kept += cnt[0] + cnt[1] + cnt[2] + cnt[3];  // this group's keeps; summed over groups below
+    }
+  }
+  // n_kept is per group (uniform within 16 lanes): lanes 0, 16, 32, 48 add theirs
+  if (c16 == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
+// --------------------------------------------------------------------------
+// Ranking pass of one probe-rank band, block form with a query ring. A block
+// of 4 waves takes items (list, up to 4 x 16 T rows, up to QC of the band's
+// queries of the list), dealt round-robin; wave w keeps rows [16 T w, 16 T (w +
+// 1)) of the item in registers as MFMA B operands. The item's queries stream
+// through a two-stage LDS buffer, 16 per step: the codes of the fragment LA
+// steps ahead are loaded into registers (each wave one 1 KB piece of every
+// fragment, gathered by query id, ids loaded a further step ahead), written to
+// the stage the step before use, and read by all four waves, so every query
+// byte crosses L2 once per block and the gather latency hides behind LA - 1
+// steps of MFMAs. The hot and exact tests and the pool appends are those of
+// k_rank_wave.
+template <int KC, int T>
+__global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
+  constexpr uint32_t P = 64 * KC;
+  constexpr int LA = 4;              // fragments loaded ahead of use
+  constexpr int NR = LA - 1;         // register ring: fragments g + 1 .. g + LA - 1
+  constexpr int NPW = (KC + 3) / 4;  // 1 KB pieces per wave per fragment
+  __shared__ __attribute__((aligned(16))) int8_t sA[2][KC * 1024];
+  __shared__ __attribute__((aligned(16))) int32_t sH[2][16];
+  __shared__ int32_t sN[2][16];
+  __shared__ uint32_t sQ[2][16];
+  const int lane = lane_id(), w = threadIdx.x >> 6, g16 = lane >> 4, c16 = lane & 15;
+  const uint64_t gm = 0xffffull << (16 * g16);
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  uint32_t n_kept = 0;
+  for (uint32_t it = ib + blockIdx.x; it < ie; it += gridDim.x) {
+    const uint4 d = a.items[it];
+    const uint32_t nq = d.y, nf = (nq + 15) / 16;
+    const uint32_t *ql = a.qlist + d.x;
+    const uint32_t r0 = d.z + static_cast<uint32_t>(w) * 16 * T;
+    const uint32_t rows = r0 < d.w ? min(16u * T, d.w - r0) : 0u;
+    // the wave's rows: B operands, norms and ids (a wave past the list's end
+    // reads the last row and keeps nothing)
+    v4i b[T * KC];
+    int32_t xn[T];
+    uint32_t xid[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const uint32_t row = min(r0 + 16 * t + c16, d.w - 1);
+      const int8_t *rp = a.plane + static_cast<size_t>(row) * P + 16 * g16;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) b[t * KC + kc] = load16(rp + 64 * kc);
+      xn[t] = a.xnorm[row];
+      xid[t] = a.ids[row];
+    }
+    // producer: this wave's pieces of fragment f (query id q of lane c16),
+    // and for wave 0 the fragment's norms and bounds
+    auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
+    struct Frag {
+      v4i p[NPW];
+      int32_t qn, lb;
+      uint32_t q;
+    };
+    auto load_frag = [&](uint32_t q) {
+      Frag r;
+      const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
+#pragma unroll
+      for (int j = 0; j < NPW; ++j) {
+        const int pc = w + 4 * j;
+        r.p[j] = pc < KC ? load16(src + 64 * pc) : v4i{0, 0, 0, 0};
+      }
+      r.q = q;
+      r.qn = w == 0 ? a.qnorm[q] : 0;
+      r.lb = w == 0 ? a.lbound[q] : 0;
+      return r;
+    };
+    auto store_frag = [&](const Frag &r, int st) {
+#pragma unroll
+      for (int j = 0; j < NPW; ++j) {
+        const int pc = w + 4 * j;
+        if (pc < KC) *reinterpret_cast<v4i *>(&sA[st][pc * 1024 + lane * 16]) = r.p[j];
+      }
+      if (w == 0 && lane < 16) {
+        sH[st][lane] = r.qn - r.lb;
+        sN[st][lane] = r.qn;
+        sQ[st][lane] = r.q;
+      }
+    };
+    // prologue: fragment 0 to stage 0, fragments 1 .. LA - 1 into the ring
+    Frag ring[NR];
+    uint32_t qnext = 0;
+    {
+      uint32_t q0s[LA];
+#pragma unroll
+      for (int i = 0; i < LA; ++i) q0s[i] = qid_of(min(static_cast<uint32_t>(i), nf - 1));
+      qnext = qid_of(min(static_cast<uint32_t>(LA), nf - 1));
+      const Frag f0 = load_frag(q0s[0]);
+#pragma unroll
+      for (int i = 0; i < NR; ++i) ring[i] = load_frag(q0s[i + 1]);
+      store_frag(f0, 0);
+    }
+    __syncthreads();
+    for (uint32_t g = 0; g < nf; ++g) {
+      const int st = g & 1;
+      // ahead: fragment g + LA's codes (ids loaded last step), g + LA + 1's ids
+      Frag fn = ring[NR - 1];
+      if (g + LA < nf) fn = load_frag(qnext);
+      if (g + LA + 1 < nf) qnext = qid_of(g + LA + 1);
+      if (rows) {
+        v4i A[KC];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) A[kc] = *reinterpret_cast<const v4i *>(&sA[st][kc * 1024 + lane * 16]);
+        const v4i hv = *reinterpret_cast<const v4i *>(&sH[st][4 * g16]);
+        v4i acc[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+          for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
+        int32_t h4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h4[r] = 16 * g + 4 * g16 + r < nq ? hv[r] : INT32_MAX;
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int32_t m = acc[0][r] - (xn[0] >> 1);
+#pragma unroll
+          for (int t = 1; t < T; ++t) m = max(m, acc[t][r] - (xn[t] >> 1));
+          any |= m >= (h4[r] >> 1);
+        }
+        if (ballot(any)) {
+          auto keep = [&](int t, int r) {
+            return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r];
+          };
+          uint32_t cnt[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            cnt[r] = 0;
+#pragma unroll
+            for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
+          }
+          const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
+          const uint32_t qmine = sQ[st][4 * g16 + (c16 & 3)];
+          uint32_t base = 0;
+          if (c16 < 4 && mine)
+            base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
+#pragma unroll
+          for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int src = 4 * g16 + r;
+            uint32_t off = __shfl(base, 16 * g16 + r);
+            const uint32_t q = sQ[st][src];
+            const int32_t qnr = sN[st][src];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+              const bool bit = keep(t, r);
+              const uint64_t m = ballot(bit) & gm;
+              if (bit) {
+                const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+                const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
+                if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+                else atomicOr(&a.state[0], 1u);
+              }
+              off += static_cast<uint32_t>(__popcll(m));
+            }
+          }
+          n_kept += cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        }
+      }
+      // fragment g + 1 into the other stage, then the ring moves on
+      if (g + 1 < nf) store_frag(ring[0], st ^ 1);
+#pragma unroll
+      for (int i = 0; i + 1 < NR; ++i) ring[i] = ring[i + 1];
+      ring[NR - 1] = fn;
+      __syncthreads();
+    }
+  }
+  if (c16 == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
 // After a band: each query's pool back to its S smallest keys, ascending, and
 // the query's threshold to the S-th of them (an upper bound of its final S-th
 // key), with the integer l2 bound of that threshold for the next band's
@@ -494,8 +829,23 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
     const uint32_t i = lane + 64 * r;
     t.e[r] = i < sorted ? pq[i] : kNoKey;
   }
+  // four batches of 64 new keys in flight at a time
+  uint64_t kb[4];
+  auto fetch = [&](uint32_t b0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = b0 + 64 * u + lane;
+      kb[u] = i < n ? pq[i] : kNoKey;
+    }
+  };
+  fetch(sorted);
   for (uint32_t b = sorted; b < n; b += 64) {
-    const uint64_t k = b + lane < n ? pq[b + lane] : kNoKey;
+    const uint32_t u = ((b - sorted) / 64) & 3;
+    uint64_t k = kb[0];
+#pragma unroll
+    for (int v = 1; v < 4; ++v)
+      if (u == static_cast<uint32_t>(v)) k = kb[v];
+    if (u == 3) fetch(b + 64);
     const uint64_t mk = ballot(k != kNoKey);
     if (__popcll(mk) <= 4) {
       uint64_t mm = mk;
@@ -620,12 +970,24 @@ __global__ __launch_bounds__(256) void k_refine(RefineArgs a) {
     const uint32_t row = sok ? a.id_row[id] : 0xffffffffu;
     const bool rok = sok && row != 0xffffffffu;
     v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
-    for (uint32_t kc = 0; kc < a.dc; kc += 64) {
-      const v4i xh = rok ? load16(a.hi + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-      const v4i xl = rok ? load16(a.lo + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-      const v4i b = col == 0 ? load16(qh + kc + koff) : (col == 1 ? load16(ql + kc + koff) : v4i{0, 0, 0, 0});
-      ah = mfma(xh, b, ah);
-      al = mfma(xl, b, al);
+    // RB chunks of 64 bytes in flight at once: the rows are random gathers
+    constexpr int RB = 8;
+    for (uint32_t kc0 = 0; kc0 < a.dc; kc0 += 64 * RB) {
+      v4i xh[RB], xl[RB], bq[RB];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        const uint32_t kc = kc0 + 64 * u;
+        const bool in = kc < a.dc;
+        xh[u] = rok && in ? load16(a.hi + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
+        xl[u] = rok && in ? load16(a.lo + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
+        bq[u] = in && col == 0 ? load16(qh + kc + koff)
+                               : (in && col == 1 ? load16(ql + kc + koff) : v4i{0, 0, 0, 0});
+      }
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        ah = mfma(xh[u], bq[u], ah);
+        al = mfma(xl[u], bq[u], al);
+      }
     }
     // survivor i = lane (< 16): C[i][0] in lane 16 (i/4), element i % 4; C[i][1] one lane up
     const int i = lane & 15;
@@ -752,6 +1114,53 @@ bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, 
     KC_CASE(1) KC_CASE(2) KC_CASE(3) KC_CASE(4) KC_CASE(5) KC_CASE(6) KC_CASE(7) KC_CASE(8)
     KC_CASE(9) KC_CASE(10) KC_CASE(11) KC_CASE(12) KC_CASE(13) KC_CASE(14) KC_CASE(15) KC_CASE(16)
 #undef KC_CASE
+    default:
+      return false;
+  }
+}
+
+// Rows per wave of the wave-independent ranking kernel: 16 T, with T one of
+// two register budgets per ranking width (B operands of 32 or 16 x 16 bytes).
+uint32_t rank_wave_rows(uint32_t P, uint32_t variant) {
+  const uint32_t kc = P / 64;
+  if (kc == 0 || kc > 16 || P % 64 || variant > 1) return 0;
+  static const uint8_t tv[17][2] = {{0, 0}, {8, 4}, {8, 4}, {8, 4}, {6, 4}, {4, 2}, {4, 2}, {4, 2}, {4, 2},
+                                     {2, 1}, {2, 1}, {2, 1}, {2, 1}, {2, 1}, {1, 1}, {1, 1}, {1, 1}};
+  return 16u * tv[kc][variant];
+}
+
+template <int KC, int T>
+static void wave_launch(const WaveArgs &a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_rank_wave<KC, T>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int KC, int T>
+static void ring_launch(const WaveArgs &a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_rank_ring<KC, T>), dim3(grid), dim3(256), 0, s, a);
+}
+
+bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                      uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                      const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                      uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                      uint32_t grid, hipStream_t s) {
+  const uint32_t T = rank_wave_rows(P, variant) / 16;
+  if (T == 0) return false;
+  const WaveArgs a{plane, xnorm, ids, qhi, dc, qnorm, lbound, qlist, item_off, items, nlist, stride, pool, pool_cnt,
+                   state, band, step_sq};
+  switch (P / 64 * 16 + T) {
+#define W_CASE(kc, t) \
+  case kc * 16 + t:   \
+    if (ring)         \
+      ring_launch<kc, t>(a, grid, s); \
+    else              \
+      wave_launch<kc, t>(a, grid, s); \
+    return true;
+    W_CASE(1, 8) W_CASE(1, 4) W_CASE(2, 8) W_CASE(2, 4) W_CASE(3, 8) W_CASE(3, 4) W_CASE(4, 6) W_CASE(4, 4)
+    W_CASE(5, 4) W_CASE(5, 2) W_CASE(6, 4) W_CASE(6, 2) W_CASE(7, 4) W_CASE(7, 2) W_CASE(8, 4) W_CASE(8, 2)
+    W_CASE(9, 2) W_CASE(9, 1) W_CASE(10, 2) W_CASE(10, 1) W_CASE(11, 2) W_CASE(11, 1) W_CASE(12, 2) W_CASE(12, 1)
+    W_CASE(13, 2) W_CASE(13, 1) W_CASE(14, 1) W_CASE(15, 1) W_CASE(16, 1)
+#undef W_CASE
     default:
       return false;
   }

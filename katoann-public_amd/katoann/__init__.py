@@ -114,6 +114,7 @@ def lib():
         L.katoann_last_work.argtypes = [P, P, i32]
         L.katoann_set_rank_rows.argtypes = [P, u32]
         L.katoann_set_band_factor.argtypes = [P, u32]
+        L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
         L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
         L.katoann_sq16_high_l2.argtypes = [P, P, P, P, f64, u32, u32, u32, P, u32]
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
@@ -368,6 +369,12 @@ class Index:
     def set_band_factor(self, log2_factor: int):
         """Probe-rank bands of the ranking pass grow by 2**log2_factor (1..4)."""
         _check(lib().katoann_set_band_factor(self._h, log2_factor))
+
+    def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 512):
+        """Ranking kernel: 0 = block items (queries staged into LDS), 1 = wave
+        items (rows held in registers, queries streamed); variant 0/1 = larger
+        or smaller row tile per wave."""
+        _check(lib().katoann_set_rank_kernel(self._h, kernel, variant, queries_per_item))
 
 
 # ---- kernel-level entry points (reference include/katoann/kernels.h:34-52)

@@ -166,3 +166,60 @@ def test_build_load_search_end_to_end(tmp_path):
     ox = O.OracleIndex.from_files(prefix)
     o = ox.search(q, k=10, nprobe=32, survivors=128)
     common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & 1) != 0)
+
+
+@pytest.fixture(scope="module")
+def wide_index(tmp_path_factory):
+    """d 1024 (prefix 256 dims, high-byte ranking over 1024): the ranking
+    widths of the embedding-shape workloads at a size the oracle runs fast."""
+    d = tmp_path_factory.mktemp("wide")
+    from katoann import builder, datasets
+    base, q = datasets.spectrum(12000, 200, 1024, alpha=1.0, seed=5)
+    prefix = str(d / "wide")
+    builder.build_numpy(base, prefix, nlist=48, kmeans_iters=4, seed=5, normalized=True)
+    return prefix, base, q
+
+
+# (kernel, variant, queries per item): block items; wave items with the large and
+# the small row tile; wave items split into many query chunks
+RANK_KERNELS = [(0, 0, 512), (1, 0, 512), (1, 1, 512), (1, 0, 16), (1, 1, 1), (2, 0, 512), (2, 1, 512), (2, 0, 20), (2, 1, 1)]
+
+
+_ORACLES = {}
+
+
+def _oracle(prefix):
+    if prefix not in _ORACLES:
+        _ORACLES[prefix] = O.OracleIndex.from_files(prefix)
+    return _ORACLES[prefix]
+
+
+@pytest.mark.parametrize("kernel,variant,qc", RANK_KERNELS)
+def test_rank_kernels_vs_oracle(mixture_index, wide_index, kernel, variant, qc):
+    """Every ranking kernel and tile gives the oracle's survivors and results,
+    for prefix ranking (P 64 and 256) and high-byte ranking (P 128 and 1024)."""
+    for name, (prefix, base, q) in (("mixture", mixture_index), ("wide", wide_index)):
+        ix = load(prefix)
+        ix.set_rank_kernel(kernel, variant, qc)
+        ox = _oracle(prefix)
+        for (k, nprobe, S, ranking) in ((10, 16, 32, "prefix"), (10, 4, 128, "prefix"), (100, 8, 100, "prefix"),
+                                        (10, 8, 32, "high")):
+            print("rank kernel", kernel, variant, qc, name, k, nprobe, S, ranking, flush=True)
+            check_against_oracle(ix, ox, q[:200], k, min(nprobe, ix.nlist), S, ranking=ranking)
+
+
+def test_rank_kernel_choice_is_invisible(mixture_index):
+    """The same batch gives bit-identical ids and distances under every ranking
+    kernel and band factor."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ref = None
+    for (kernel, variant, qc) in RANK_KERNELS:
+        for shift in (1, 2, 4):
+            ix.set_rank_kernel(kernel, variant, qc)
+            ix.set_band_factor(shift)
+            print("choice", kernel, variant, qc, shift, flush=True)
+            i, d, _ = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)
+            if ref is None:
+                ref = (i, d)
+            assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
