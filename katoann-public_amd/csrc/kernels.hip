@@ -585,30 +585,31 @@ __device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_
 // band-start keys are staged in LDS and read as broadcasts.
 constexpr int INV_LB = 256;  // lists per block
 
-template <bool FILL>
+template <bool FILL, int NB>
 __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
                                                   const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
-                                                  uint32_t B, uint32_t np, uint32_t nlist, uint32_t nb,
-                                                  uint32_t shift, uint32_t *__restrict__ part,
-                                                  const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist) {
+                                                  uint32_t B, uint32_t np, uint32_t nlist, uint32_t shift,
+                                                  uint32_t *__restrict__ part, const uint32_t *__restrict__ qoff,
+                                                  uint32_t *__restrict__ qlist) {
+  constexpr uint32_t nb = NB;
   __shared__ uint64_t s_kend[INV_QT];
-  __shared__ uint64_t s_kb[INV_MAXB][INV_QT];
+  __shared__ uint64_t s_kb[NB][INV_QT];
   const uint32_t qt = blockIdx.y, q0 = qt * INV_QT, c = blockIdx.x * INV_LB + threadIdx.x;
   const uint32_t nslots = nb * nlist;
   if (threadIdx.x < INV_QT) {
     const InvQuery r = inv_query(probes, probe_d, npq, q0 + threadIdx.x, B, np, nb, shift);
     s_kend[threadIdx.x] = r.kend;
 #pragma unroll
-    for (int b = 0; b < INV_MAXB; ++b) s_kb[b][threadIdx.x] = r.kb[b];
+    for (int b = 0; b < NB; ++b) s_kb[b][threadIdx.x] = r.kb[b];
   }
   __syncthreads();
   if (c >= nlist) return;
   const uint32_t nq = min(static_cast<uint32_t>(INV_QT), B - q0);
-  uint32_t acc[INV_MAXB];
+  uint32_t acc[NB];
 #pragma unroll
-  for (int b = 0; b < INV_MAXB; ++b) {
+  for (int b = 0; b < NB; ++b) {
     const size_t slot = static_cast<size_t>(b) * nlist + c;
-    acc[b] = (FILL && static_cast<uint32_t>(b) < nb) ? qoff[slot] + part[static_cast<size_t>(qt) * nslots + slot] : 0u;
+    acc[b] = FILL ? qoff[slot] + part[static_cast<size_t>(qt) * nslots + slot] : 0u;
   }
   const float *col = dist + static_cast<size_t>(q0) * nlist + c;
   constexpr int U = 8;
@@ -624,10 +625,9 @@ __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist
       if (k > s_kend[i]) continue;
       uint32_t band = 0;
 #pragma unroll
-      for (int b = 1; b < INV_MAXB; ++b)
-        if (static_cast<uint32_t>(b) < nb) band += k >= s_kb[b][i] ? 1u : 0u;
+      for (int b = 1; b < NB; ++b) band += k >= s_kb[b][i] ? 1u : 0u;
 #pragma unroll
-      for (int b = 0; b < INV_MAXB; ++b)
+      for (int b = 0; b < NB; ++b)
         if (band == static_cast<uint32_t>(b)) {
           if (FILL) qlist[acc[b]] = q0 + i;
           ++acc[b];
@@ -635,7 +635,8 @@ __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist
     }
   }
   if (!FILL)
-    for (uint32_t b = 0; b < nb; ++b) part[static_cast<size_t>(qt) * nslots + static_cast<size_t>(b) * nlist + c] = acc[b];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) part[static_cast<size_t>(qt) * nslots + static_cast<size_t>(b) * nlist + c] = acc[b];
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -862,6 +863,24 @@ size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shif
   return qtiles * nslots + nslots + 2 * nblk;
 }
 
+template <bool FILL>
+static void inv_pass_launch(uint32_t nb, dim3 tiles, hipStream_t s, const float *dist, const uint32_t *probes,
+                            const float *probe_d, const uint32_t *npq, uint32_t B, uint32_t np, uint32_t nlist,
+                            uint32_t shift, uint32_t *part, const uint32_t *qoff, uint32_t *qlist) {
+  switch (nb) {
+#define NB_CASE(n)                                                                                              \
+  case n:                                                                                                       \
+    hipLaunchKernelGGL((k_inv_pass<FILL, n>), tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, \
+                       shift, part, qoff, qlist);                                                               \
+    break;
+    NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
+    NB_CASE(9) NB_CASE(10) NB_CASE(11) NB_CASE(12) NB_CASE(13) NB_CASE(14) NB_CASE(15) NB_CASE(16)
+#undef NB_CASE
+    default:
+      break;
+  }
+}
+
 void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
                  uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
                  uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
@@ -872,13 +891,11 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
-  hipLaunchKernelGGL(k_inv_pass<false>, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift,
-                     part, nullptr, nullptr);
+  inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, part, nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, gcnt,
                      nit, bsum);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
-  hipLaunchKernelGGL(k_inv_pass<true>, tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, nb, shift,
-                     part, qoff, qlist);
+  inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, part, qoff, qlist);
   hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr,
                      lbound);
 }

@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
         base = atomicAdd(&a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], nxt - lane);
       }
       const uint64_t le = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
-      const int h = 63 - __builtin_clzll(hm & le | 1ull);
+      const int h = 63 - __builtin_clzll((hm & le) | 1ull);
       const uint32_t pos = __shfl(base, h) + static_cast<uint32_t>(lane - h);
       if (v) {
         if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
@@ -485,9 +485,8 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
 //   dot - a >= floor(h / 2): the hot test is one add per row, a max over the
 //   rows and one compare per query. Fragments that pass it take the exact
 //   test and form the key f(qn + xn - 2 dot).
-// Kept keys go straight to the pools: one returning atomic per query of the
-// fragment (16 lanes, one round trip) reserves its slots. Items are dealt to
-// the waves round-robin (no shared counter).
+// Kept keys go to the pools through the wave's deferred stage (keep_fragment).
+// Items are dealt to the waves round-robin (no shared counter).
 struct WaveArgs {
   const int8_t *plane;       // N x P, row-major, shifted high bytes
   const int32_t *xnorm;      // N
@@ -508,12 +507,135 @@ struct WaveArgs {
   double step_sq;
 };
 
-template <int KC, int T>
-__global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
-  constexpr uint32_t P = 64 * KC;
+// --------------------------------------------------------------------------
+// Deferred pool appends of one wave. A fragment's kept keys are staged in the
+// wave's LDS region with their query slot j (0..15) and their index within the
+// slot's run; one returning atomic per query reserves the run in the query's
+// pool, and the keys are written to the pool when the wave next keeps keys (or
+// at its end), so the atomic's round trip overlaps the MFMAs in between. A
+// fragment with more than KCAP keys (the first band keeps every candidate) is
+// written at once instead.
+constexpr int KCAP = 256;
+
+struct KeepStage {
+  uint64_t *key;       // LDS: 2 x KCAP per wave
+  uint32_t *meta;      // LDS: 2 x KCAP per wave: slot j | index << 4
+  uint32_t pending;    // keys staged in region `par` (wave-uniform)
+  int par;
+  uint32_t base, q;    // lane 16 g + r (r < 4): pool base and query of slot 4 g + r
+};
+
+__device__ __forceinline__ void keep_drain(KeepStage &ks, const WaveArgs &a) {
+  const int lane = lane_id();
+  for (uint32_t e0 = 0; e0 < ks.pending; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    const bool v = e < ks.pending;
+    const uint32_t m = v ? ks.meta[ks.par * KCAP + e] : 0u;
+    const uint64_t key = v ? ks.key[ks.par * KCAP + e] : 0ull;
+    const int src = 16 * ((m & 15) >> 2) + (m & 3);
+    const uint32_t pos = __shfl(ks.base, src) + (m >> 4);
+    const uint32_t q = __shfl(ks.q, src);
+    if (v) {
+      if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+      else atomicOr(&a.state[0], 1u);
+    }
+  }
+  ks.pending = 0;
+}
+
+// Exact test of a fragment that passed the hot test, and the appends of its
+// keys. acc[t][r]: dot of row 16 t + c16 with query slot 4 g16 + r; h4[r]:
+// qn - L of that query (INT32_MAX past the item); qslot(j) / qnorm_slot(j): id
+// and norm of query slot j (lane-uniform calls). Returns the keys kept.
+template <int T, class QS, class QN>
+__device__ __forceinline__ uint32_t keep_fragment(KeepStage &ks, const WaveArgs &a, v4i (&acc)[T],
+                                                  const int32_t (&xn)[T], const uint32_t (&xid)[T], uint32_t rows,
+                                                  const int32_t (&h4)[4], QS qslot, QN qnorm_slot) {
   const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
   const uint64_t gm = 0xffffull << (16 * g16);
   const uint64_t lt_mask = (1ull << lane) - 1ull;
+  auto keep = [&](int t, int r) { return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r]; };
+  uint32_t cnt[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    cnt[r] = 0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
+  }
+  // slot j = 4 g16 + r: run offsets in the stage (exclusive prefix over j)
+  const uint32_t G = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+  const uint32_t G0 = __shfl(G, 0), G1 = __shfl(G, 16), G2 = __shfl(G, 32), G3 = __shfl(G, 48);
+  const uint32_t total = G0 + G1 + G2 + G3;
+  uint32_t so[4];
+  so[0] = (g16 > 0 ? G0 : 0u) + (g16 > 1 ? G1 : 0u) + (g16 > 2 ? G2 : 0u);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) so[r] = so[r - 1] + cnt[r - 1];
+  const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
+  const uint32_t qmine = qslot(4 * g16 + (c16 & 3));
+  // the tests again rather than 4 T masks kept live from the count
+#pragma unroll
+  for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
+  if (total > static_cast<uint32_t>(KCAP)) {
+    keep_drain(ks, a);
+    uint32_t base = 0;
+    if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = 4 * g16 + r;
+      uint32_t off = __shfl(base, 16 * g16 + r);
+      const uint32_t q = qslot(j);
+      const int32_t qnr = qnorm_slot(j);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const bool bit = keep(t, r);
+        const uint64_t m = ballot(bit) & gm;
+        if (bit) {
+          const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+          const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
+          if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+          else atomicOr(&a.state[0], 1u);
+        }
+        off += static_cast<uint32_t>(__popcll(m));
+      }
+    }
+    return total;
+  }
+  const int reg = ks.par ^ 1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int j = 4 * g16 + r;
+    uint32_t off = so[r];
+    const int32_t qnr = qnorm_slot(j);
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const bool bit = keep(t, r);
+      const uint64_t m = ballot(bit) & gm;
+      if (bit) {
+        const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+        ks.key[reg * KCAP + pos] = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
+        ks.meta[reg * KCAP + pos] = static_cast<uint32_t>(j) | ((pos - so[r]) << 4);
+      }
+      off += static_cast<uint32_t>(__popcll(m));
+    }
+  }
+  uint32_t base = 0;
+  if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
+  keep_drain(ks, a);
+  ks.par = reg;
+  ks.pending = total;
+  ks.base = base;
+  ks.q = qmine;
+  return total;
+}
+
+
+template <int KC, int T>
+__global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
+  constexpr uint32_t P = 64 * KC;
+  __shared__ uint64_t s_key[4][2 * KCAP];
+  __shared__ uint32_t s_meta[4][2 * KCAP];
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  KeepStage ks{s_key[threadIdx.x >> 6], s_meta[threadIdx.x >> 6], 0u, 0, 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
   // items dealt round-robin over the grid's waves: a shared queue counter
@@ -540,24 +662,28 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
     auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
     uint32_t qid1 = qid_of(0), qid2 = nf > 1 ? qid_of(1) : qid1;
     v4i An[KC];
-    int32_t qnn, hn;
-    auto load_q = [&](uint32_t q, v4i(&A)[KC], int32_t &qn, int32_t &h) {
+    int32_t qnn, lbn;
+    // norms and bounds stay raw until the fragment is ranked: using them here
+    // would wait for the loads just issued
+    auto load_q = [&](uint32_t q, v4i(&A)[KC], int32_t &qn, int32_t &lb) {
       const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) A[kc] = load16(src + 64 * kc);
       qn = a.qnorm[q];
-      h = qn - a.lbound[q];
+      lb = a.lbound[q];
     };
-    load_q(qid1, An, qnn, hn);
+    load_q(qid1, An, qnn, lbn);
     for (uint32_t f = 0; f < nf; ++f) {
       v4i A[KC];
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) A[kc] = An[kc];
-      const int32_t qn = qnn, h = hn;
+      const int32_t qn = qnn, lb = lbn;
       const uint32_t qid = qid1;
       qid1 = qid2;
-      if (f + 1 < nf) load_q(qid1, An, qnn, hn);
+      if (f + 1 < nf) load_q(qid1, An, qnn, lbn);
       if (f + 2 < nf) qid2 = qid_of(f + 2);
+      // lane c16's query: h = qn - L, or INT32_MAX past the item
+      const int32_t h = 16 * f + c16 < nq ? qn - lb : INT32_MAX;
       v4i acc[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) acc[t] = v4i{0, 0, 0, 0};
@@ -565,14 +691,10 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
       for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
         for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
-      // h of the lane's 4 queries 4 g16 + r (no query past the item)
+      // h of the lane's 4 queries 4 g16 + r
       int32_t h4[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t j = 4 * g16 + r;
-        const int32_t hv = __shfl(h, static_cast<int>(j));
-        h4[r] = 16 * f + j < nq ? hv : INT32_MAX;
-      }
+      for (int r = 0; r < 4; ++r) h4[r] = __shfl(h, 4 * g16 + r);
       bool any = false;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -582,48 +704,13 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
         any |= m >= (h4[r] >> 1);
       }
       if (!ballot(any)) continue;
-      // exact test, per-query counts in the lane's group, one reservation per query
-      auto keep = [&](int t, int r) {
-        return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r];
-      };
-      uint32_t cnt[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        cnt[r] = 0;
-#pragma unroll
-        for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
-      }
-      const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
-      const uint32_t qmine = __shfl(qid, 4 * g16 + (c16 & 3));
-      uint32_t base = 0;
-      if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
-      // the tests again rather than 4 T masks kept live from the count
-#pragma unroll
-      for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int src = 4 * g16 + r;
-        uint32_t off = __shfl(base, 16 * g16 + r);
-        const uint32_t q = __shfl(qid, src);
-        const int32_t qnr = __shfl(qn, src);
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const bool bit = keep(t, r);
-          const uint64_t m = ballot(bit) & gm;
-          if (bit) {
-            const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-            const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
-            if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
-            else atomicOr(&a.state[0], 1u);
-          }
-          off += static_cast<uint32_t>(__popcll(m));
-        }
-      }
-      n_kept += cnt[0] + cnt[1] + cnt[2] + cnt[3];  // this group's keeps; summed over groups below
+      n_kept += keep_fragment<T>(
+          ks, a, acc, xn, xid, rows, h4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
+          [&](int j) { return __shfl(qn, j); });
     }
   }
-  // n_kept is per group (uniform within 16 lanes): lanes 0, 16, 32, 48 add theirs
-  if (c16 == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+  keep_drain(ks, a);
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
 // --------------------------------------------------------------------------
@@ -648,9 +735,10 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
   __shared__ __attribute__((aligned(16))) int32_t sH[2][16];
   __shared__ int32_t sN[2][16];
   __shared__ uint32_t sQ[2][16];
+  __shared__ uint64_t s_key[4][2 * KCAP];
+  __shared__ uint32_t s_meta[4][2 * KCAP];
   const int lane = lane_id(), w = threadIdx.x >> 6, g16 = lane >> 4, c16 = lane & 15;
-  const uint64_t gm = 0xffffull << (16 * g16);
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  KeepStage ks{s_key[w], s_meta[w], 0u, 0, 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
   for (uint32_t it = ib + blockIdx.x; it < ie; it += gridDim.x) {
@@ -749,45 +837,9 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
           for (int t = 1; t < T; ++t) m = max(m, acc[t][r] - (xn[t] >> 1));
           any |= m >= (h4[r] >> 1);
         }
-        if (ballot(any)) {
-          auto keep = [&](int t, int r) {
-            return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r];
-          };
-          uint32_t cnt[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            cnt[r] = 0;
-#pragma unroll
-            for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
-          }
-          const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
-          const uint32_t qmine = sQ[st][4 * g16 + (c16 & 3)];
-          uint32_t base = 0;
-          if (c16 < 4 && mine)
-            base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
-#pragma unroll
-          for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int src = 4 * g16 + r;
-            uint32_t off = __shfl(base, 16 * g16 + r);
-            const uint32_t q = sQ[st][src];
-            const int32_t qnr = sN[st][src];
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-              const bool bit = keep(t, r);
-              const uint64_t m = ballot(bit) & gm;
-              if (bit) {
-                const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-                const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
-                if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
-                else atomicOr(&a.state[0], 1u);
-              }
-              off += static_cast<uint32_t>(__popcll(m));
-            }
-          }
-          n_kept += cnt[0] + cnt[1] + cnt[2] + cnt[3];
-        }
+        if (ballot(any))
+          n_kept += keep_fragment<T>(ks, a, acc, xn, xid, rows, h4, [&](int j) { return sQ[st][j]; },
+                                     [&](int j) { return sN[st][j]; });
       }
       // fragment g + 1 into the other stage, then the ring moves on
       if (g + 1 < nf) store_frag(ring[0], st ^ 1);
@@ -797,7 +849,8 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
       __syncthreads();
     }
   }
-  if (c16 == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+  keep_drain(ks, a);
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
 // After a band: each query's pool back to its S smallest keys, ascending, and
@@ -814,11 +867,28 @@ struct UpdateArgs {
   double step_sq;
 };
 
+// One block of 4 waves per query: wave w merges batches w, w + 4, ... of the
+// new keys into its own sorted list (wave 0 starts from the previous list);
+// once a wave's list holds S keys, new keys above its S-th are dropped before
+// any sorting. Wave 0 then merges the other three lists.
+template <int SC>
+__device__ inline void list_merge_from(TopList<SC> &t, const uint64_t *u, uint32_t S) {
+  // t ascending, u ascending in LDS: t[i] = min(t[i], u[SC - 1 - i]) is a
+  // bitonic sequence holding the SC smallest of both
+  const int lane = lane_id();
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) t.e[r] = umin64(t.e[r], u[SC - 1 - (64 * r + lane)]);
+  list_bitonic_merge<SC>(t);
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r)
+    if (static_cast<uint32_t>(lane + 64 * r) >= S) t.e[r] = kNoKey;
+}
+
 template <int SC>
 __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
-  const int lane = lane_id();
-  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= a.B) return;
+  __shared__ uint64_t part[3][SC];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x;
   const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride),
                  sorted = a.pool_sorted[q];
   if (n == sorted) return;
@@ -827,26 +897,18 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
 #pragma unroll
   for (int r = 0; r < SC / 64; ++r) {
     const uint32_t i = lane + 64 * r;
-    t.e[r] = i < sorted ? pq[i] : kNoKey;
+    t.e[r] = w == 0 && i < sorted ? pq[i] : kNoKey;
   }
-  // four batches of 64 new keys in flight at a time
-  uint64_t kb[4];
-  auto fetch = [&](uint32_t b0) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t i = b0 + 64 * u + lane;
-      kb[u] = i < n ? pq[i] : kNoKey;
-    }
-  };
-  fetch(sorted);
-  for (uint32_t b = sorted; b < n; b += 64) {
-    const uint32_t u = ((b - sorted) / 64) & 3;
-    uint64_t k = kb[0];
-#pragma unroll
-    for (int v = 1; v < 4; ++v)
-      if (u == static_cast<uint32_t>(v)) k = kb[v];
-    if (u == 3) fetch(b + 64);
+  uint64_t cap = w == 0 && sorted >= a.S ? top_get<SC>(t, a.S - 1) : kNoKey;  // keys above it cannot stay
+  // this wave's batches, the next one in flight
+  uint32_t b = sorted + 64 * w;
+  uint64_t kn = b + lane < n ? pq[b + lane] : kNoKey;
+  for (; b < n; b += 256) {
+    uint64_t k = kn;
+    kn = b + 256 + lane < n ? pq[b + 256 + lane] : kNoKey;
+    if (k >= cap) k = kNoKey;
     const uint64_t mk = ballot(k != kNoKey);
+    if (!mk) continue;
     if (__popcll(mk) <= 4) {
       uint64_t mm = mk;
       while (mm) {
@@ -857,7 +919,17 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
     } else {
       top_merge<SC>(t, k, a.S);
     }
+    const uint64_t last = top_get<SC>(t, a.S - 1);
+    if (last != kNoKey) cap = last;
   }
+  if (w > 0) {
+#pragma unroll
+    for (int r = 0; r < SC / 64; ++r) part[w - 1][lane + 64 * r] = t.e[r];
+  }
+  __syncthreads();
+  if (w > 0) return;
+#pragma unroll
+  for (int v = 0; v < 3; ++v) list_merge_from<SC>(t, part[v], a.S);
   const uint32_t cnt = top_count<SC>(t);
 #pragma unroll
   for (int r = 0; r < SC / 64; ++r) {
@@ -897,15 +969,6 @@ struct RefineArgs {
   uint32_t *nsurv;           // B or null
 };
 
-template <int SC>
-__device__ inline uint64_t wave_max_list(const TopList<SC> &t) {
-  uint64_t m = 0;
-#pragma unroll
-  for (int r = 0; r < SC / 64; ++r) m = umax64(m, t.e[r] == kNoKey ? 0ull : t.e[r]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = umax64(m, __shfl_xor(m, o));
-  return m;
-}
 
 // Final survivors, exact distances, arrival rule, output (src/index.cpp:1046-1127):
 //  1. the S smallest keys of the query's pool (exactly the S smallest ranking
@@ -915,7 +978,8 @@ __device__ inline uint64_t wave_max_list(const TopList<SC> &t) {
 //     and low bytes as columns 0 and 1), combined in int64;
 //  3. the reference's arrival rule: survivors in descending (rank, id) order
 //     enter a k-entry max-heap of (distance, id) while it has room, then
-//     replace its maximum iff their distance is strictly smaller;
+//     replace its maximum iff their distance is strictly smaller (evaluated in
+//     closed form, below);
 //  4. output ascending by (distance, id), padded with FLT_MAX / UINT32_MAX
 //     (src/index.cpp:384-398).
 template <int SC>
@@ -1020,42 +1084,50 @@ __global__ __launch_bounds__(256) void k_refine(RefineArgs a) {
   }
   wave_fence();
 
-  TopList<SC> h;
+  // The arrival rule in closed form. Survivors arrive in descending rank order
+  // (here: index ns - 1 first) and enter a K-entry max-heap of (dist, id) that
+  // admits a new one while it has room, else iff its dist is strictly below
+  // the heap's largest. The heap's dists are always the K smallest of those
+  // arrived so far, so with M = the K-th smallest dist of all survivors the
+  // final heap is: every survivor with dist < M, and of those with dist == M,
+  // the ones admitted (fewer than K earlier arrivals with dist <= M) minus the
+  // largest ids evicted once the heap's largest dist is M, i.e. the admitted
+  // ones with the smallest ids. Sorting that set by (dist, id) and taking K
+  // entries gives exactly the heap's output order.
+  TopList<SC> e;
 #pragma unroll
-  for (int r = 0; r < SC / 64; ++r) h.e[r] = kNoKey;
-  uint32_t nh = 0;
-  uint64_t mx = 0;
-  for (int t = static_cast<int>(ns) - 1; t >= 0; --t) {
-    const uint64_t c = ex[w][t];
-    if (c == kNoKey) continue;
-    if (nh < a.K) {
-#pragma unroll
-      for (int r = 0; r < SC / 64; ++r)
-        if (static_cast<uint32_t>(r) == nh / 64 && lane == static_cast<int>(nh % 64)) h.e[r] = c;
-      ++nh;
-      if (nh == a.K) mx = wave_max_list<SC>(h);
-    } else if (key_float(c) < key_float(mx)) {
-      bool done = false;
-#pragma unroll
-      for (int r = 0; r < SC / 64; ++r) {
-        const uint64_t hit = ballot(h.e[r] == mx);
-        if (!done && hit) {
-          const int hl2 = __ffsll(static_cast<long long>(hit)) - 1;
-          if (lane == hl2) h.e[r] = c;
-          done = true;
-        }
-      }
-      mx = wave_max_list<SC>(h);
-    }
+  for (int r = 0; r < SC / 64; ++r) {
+    const uint32_t i = lane + 64 * r;
+    e.e[r] = i < ns ? ex[w][i] : kNoKey;
   }
-  list_sort<SC>(h);
+  TopList<SC> srt = e;
+  list_sort<SC>(srt);
+  if (top_count<SC>(srt) > a.K) {
+    const uint32_t mb = static_cast<uint32_t>(top_get<SC>(srt, a.K - 1) >> 32);  // bits of M (dists >= 0)
+    uint32_t above = 0;  // survivors with dist <= M in higher registers (earlier arrivals)
+    const uint64_t higher = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
+#pragma unroll
+    for (int r = SC / 64 - 1; r >= 0; --r) {
+      const uint64_t k = e.e[r];
+      const uint32_t fb = static_cast<uint32_t>(k >> 32);
+      const bool valid = k != kNoKey;
+      const uint64_t le = ballot(valid && fb <= mb);
+      const uint32_t pre = above + static_cast<uint32_t>(__popcll(le & higher));
+      const bool in = valid && (fb < mb || (fb == mb && pre < a.K));
+      e.e[r] = in ? k : kNoKey;
+      above += static_cast<uint32_t>(__popcll(le));
+    }
+    list_sort<SC>(e);
+  } else {
+    e = srt;
+  }
 #pragma unroll
   for (int r = 0; r < SC / 64; ++r) {
     const uint32_t g = lane + 64 * r;
     if (g < a.K) {
-      const bool ok = g < nh;
-      a.out_ids[static_cast<size_t>(q) * a.K + g] = ok ? static_cast<uint32_t>(h.e[r]) : 0xffffffffu;
-      a.out_dists[static_cast<size_t>(q) * a.K + g] = ok ? key_float(h.e[r]) : FLT_MAX;
+      const bool ok = e.e[r] != kNoKey;
+      a.out_ids[static_cast<size_t>(q) * a.K + g] = ok ? static_cast<uint32_t>(e.e[r]) : 0xffffffffu;
+      a.out_dists[static_cast<size_t>(q) * a.K + g] = ok ? key_float(e.e[r]) : FLT_MAX;
     }
   }
 }
@@ -1169,7 +1241,7 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
                    uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s) {
   UpdateArgs a{pool, pool_cnt, pool_sorted, thr, lbound, B, S, stride, static_cast<int64_t>(P) * 65025, step_sq};
-  const dim3 grid((B + 3) / 4);
+  const dim3 grid(B);
   switch (refine_cap(S)) {
     case 64: hipLaunchKernelGGL(k_update<64>, grid, dim3(256), 0, s, a); return true;
     case 128: hipLaunchKernelGGL(k_update<128>, grid, dim3(256), 0, s, a); return true;
