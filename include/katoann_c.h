@@ -180,6 +180,11 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
  * of 4 x 16 T rows whose 4 waves share the queries through an LDS ring;
  * variant 0 or 1 picks the larger or smaller T. Default: 1, variant 0, 512. */
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item);
+/* Graph replay (no reference counterpart): each search context captures its
+ * kernel sequence (normalisation through top-k) into a hipGraph once and
+ * replays it while shapes, parameters and buffers stay the same (default on;
+ * with stage timing on, the timing events are nodes of the graph). */
+int katoann_set_graphs(katoann_index *h, int32_t on);
 
 /* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
  * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of

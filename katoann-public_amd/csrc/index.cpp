@@ -87,12 +87,15 @@ struct SearchCtx {
       totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items, mx16, mh1,
       mh2, mz;
   std::vector<uint32_t> host_totals;
+  hipGraphExec_t exec = nullptr;  // the last captured pass and its launch arguments
+  std::vector<uint64_t> gkey;
   SearchCtx() {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     for (auto &e : bev) HIPCHECK(hipEventCreate(&e));
   }
   ~SearchCtx() {
+    if (exec) (void)hipGraphExecDestroy(exec);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto &e : bev)
@@ -115,6 +118,8 @@ struct katoann_index {
   uint32_t rank_rows = 1024;
   uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
   uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 512;
+  std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
+  uint64_t gen = 0;                // bumped by every load: captured graphs name the old buffers
   std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
   uint32_t tile_prefix_rows = 0;
 
@@ -129,6 +134,7 @@ struct katoann_index {
   uint64_t work[4] = {};
 
   void reset() {
+    ++gen;
     dim = nlist = npts = dc = head_dims = 0;
     normalized = rotated = false;
     batched = true;
@@ -552,6 +558,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                             dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     npq = c.npq.as<uint32_t>();
   }
+  if (model) npq = c.npq.as<uint32_t>();  // written by the model's kernels (also on graph replays)
   auto mark = [&](int i) {
     if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
   };
@@ -563,11 +570,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
 
   // pool: S slots plus one band's kept keys per query; on overflow the batch
   // runs again with room for every candidate of the query
-  auto run = [&](uint64_t allowance) {
-    const uint32_t stride = static_cast<uint32_t>(std::min<uint64_t>(S + allowance, 0xffffffffu));
-    c.pool.ensure(std::max<uint64_t>(static_cast<uint64_t>(stride) * B, 1) * 8);
-    HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
-                            dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  // the kernels of one pass (normalisation through top-k), launched eagerly
+  // or captured once into a graph and replayed while every launch argument
+  // stays the same
+  auto launches = [&](uint32_t stride) {
     mark(0);
     if (x && x->normalize) katoann_hip::launch_normalize(c.q.as<float>(), B, d, s);
     yq = c.q.as<float>();
@@ -597,16 +603,13 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                     c.mz.as<float>(), nullptr, 1, s);
       katoann_hip::launch_nprobe_of(c.mz.as<float>(), B, model->y_mean, model->y_std, model->margin,
                                     std::max(1u, std::min(model->nmin, np)), np, c.npq.as<uint32_t>(), s);
-      npq = c.npq.as<uint32_t>();
     }
     mark(3);
     if (coarse_only) {
       mark(4);
       mark(5);
       mark(6);
-      HIPCHECK(hipGetLastError());
-      HIPCHECK(hipStreamSynchronize(s));
-      return true;
+      return;
     }
     katoann_hip::launch_plan(c.dist.as<float>(), c.probes.as<uint32_t>(), c.probe_d.as<float>(), npq, B, np,
                              ix.list_off_d.as<uint64_t>(), nlist, qc, rt, c.pscratch.as<uint32_t>(),
@@ -648,7 +651,57 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                     (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s))
       throw std::invalid_argument("unsupported refinement shape");
     mark(6);
+  };
+  const bool graphs = mix.graphs.load();
+  auto run = [&](uint64_t allowance) {
+    const uint32_t stride = static_cast<uint32_t>(std::min<uint64_t>(S + allowance, 0xffffffffu));
+    c.pool.ensure(std::max<uint64_t>(static_cast<uint64_t>(stride) * B, 1) * 8);
+    HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
+                            dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (graphs) {
+      // everything a launch reads: shapes, scalars, every buffer, the stream
+      std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, qc,
+                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, grid, ix.gen,
+                                   ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
+                                   reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
+                                   x && x->survivors, x && x->n_survivors, x && x->codes, timing};
+      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
+                              &c.npq, &c.gcnt, &c.qoff, &c.pscratch, &c.item_off, &c.qlist, &c.totals, &c.pool_cnt,
+                              &c.pool_sorted, &c.thr, &c.lbound, &c.state, &c.pool, &c.surv, &c.nsurv, &c.codes,
+                              &c.items, &c.mx16, &c.mh1, &c.mh2, &c.mz})
+        key.push_back(reinterpret_cast<uint64_t>(b->p));
+      if (model) {
+        uint64_t words[(sizeof(katoann_nprobe_model) + 7) / 8] = {};
+        std::memcpy(words, model, sizeof(katoann_nprobe_model));
+        key.insert(key.end(), std::begin(words), std::end(words));
+      }
+      if (!c.exec || c.gkey != key) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        c.exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        try {
+          launches(stride);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHECK(hipStreamEndCapture(s, &g));
+        const hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(ie);
+        c.gkey = std::move(key);
+      }
+      HIPCHECK(hipGraphLaunch(c.exec, s));
+    } else {
+      launches(stride);
+    }
     HIPCHECK(hipGetLastError());
+    if (coarse_only) {
+      HIPCHECK(hipStreamSynchronize(s));
+      return true;
+    }
     HIPCHECK(hipMemcpyAsync(st, c.state.p, 12, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(st + 4, c.totals.p, 8, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -927,6 +980,11 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
   std::lock_guard<std::mutex> g(h->mu);
   h->band_shift = log2_factor;
   KATOANN_CATCH
+}
+
+int katoann_set_graphs(katoann_index *h, int32_t on) {
+  h->graphs.store(on != 0);
+  return KATOANN_OK;
 }
 
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item) {

@@ -115,6 +115,7 @@ def lib():
         L.katoann_set_rank_rows.argtypes = [P, u32]
         L.katoann_set_band_factor.argtypes = [P, u32]
         L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
+        L.katoann_set_graphs.argtypes = [P, i32]
         L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
         L.katoann_sq16_high_l2.argtypes = [P, P, P, P, f64, u32, u32, u32, P, u32]
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
@@ -375,6 +376,10 @@ class Index:
         items (rows held in registers, queries streamed); variant 0/1 = larger
         or smaller row tile per wave."""
         _check(lib().katoann_set_rank_kernel(self._h, kernel, variant, queries_per_item))
+
+    def set_graphs(self, on: bool = True):
+        """Replay each search's kernels from a captured hipGraph (default on)."""
+        _check(lib().katoann_set_graphs(self._h, int(on)))
 
 
 # ---- kernel-level entry points (reference include/katoann/kernels.h:34-52)

@@ -223,3 +223,24 @@ def test_rank_kernel_choice_is_invisible(mixture_index):
             if ref is None:
                 ref = (i, d)
             assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
+
+
+def test_graph_replays_match_eager(mixture_index):
+    """Searches replayed from the captured graph equal eager searches, replay
+    after replay, with other shapes and eager searches run in between."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ix.set_graphs(False)
+    ref = ix.search_ex(q, k=10, nprobe=32, survivors=64, normalize=False)[:2]
+    other = ix.search_ex(q[:77], k=5, nprobe=8, survivors=32, normalize=False)[:2]
+    ix.set_graphs(True)
+    for rep in range(4):
+        print("graph replay", rep, flush=True)
+        for _ in range(2):  # a capture, then a replay of it
+            i, d, _ = ix.search_ex(q, k=10, nprobe=32, survivors=64, normalize=False)
+            assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
+        i2, d2, _ = ix.search_ex(q[:77], k=5, nprobe=8, survivors=32, normalize=False)
+        assert np.array_equal(i2, other[0]) and np.array_equal(d2.view(np.uint32), other[1].view(np.uint32))
+        ix.set_graphs(False)
+        i3, _, _ = ix.search_ex(q[:300], k=10, nprobe=16, normalize=False)
+        ix.set_graphs(True)
