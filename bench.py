@@ -108,6 +108,8 @@ def main():
         ix.set_band_factor(args.band_shift)
     if args.rank_kernel:
         ix.set_rank_kernel(*[int(x) for x in args.rank_kernel.split(",")])
+    else:
+        ix.set_autotune(True)  # ranking tiles timed on the first search of each shape (untimed warm-up)
     t0 = time.time()
     gt = datasets.ground_truth_torch(base, queries, 10)
     log("ground truth %.1fs" % (time.time() - t0))
@@ -121,21 +123,25 @@ def main():
         ix.search_device(qptr, nq, ids_d.data_ptr(), dists_d.data_ptr(), k=k, nprobe=np_, survivors=S)
 
     sweep = []
-    ix.set_timing(True)
     grid = [(int(a), int(b)) for b in args.survivors.split(",") for a in args.nprobe.split(",")
             if int(a) <= args.nlist]
     for np_, S in grid:
         for _ in range(args.warmup):
             step(np_, S)
         torch.cuda.synchronize()
-        stage_sum = np.zeros(len(katoann.STAGES))
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(np_, S)
-            st = ix.stage_times()
-            stage_sum += np.array(list(st.values()))
+            step(np_, S)  # graph replays, no stage events
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        # the same steps again, eagerly, with HIP events around every stage and
+        # every ranking launch on the search stream (roofline durations)
+        ix.set_timing(True)
+        stage_sum = np.zeros(len(katoann.STAGES))
+        for _ in range(args.steps):
+            step(np_, S)
+            stage_sum += np.array(list(ix.stage_times().values()))
+        ix.set_timing(False)
         ids = ids_d.cpu().numpy().view(np.uint32)
         rec = recall_at(ids, gt, 10)
         # algorithmic bytes of the ranking pass: every touched list once, P + 4 bytes per row
@@ -147,10 +153,10 @@ def main():
         work = ix.last_work()
         rank_ms = stages["rank"]
         row = dict(nprobe=np_, survivors=S, ms_per_step=ms, qps=nq / (ms / 1e3), recall=rec, stages_ms=stages,
+                   tile=ix.tile_config(nq, k=k, nprobe=np_, survivors=S),
                    work=work, rank_bytes=rank_bytes, rank_gbs=rank_bytes / (rank_ms * 1e-3) / 1e9 if rank_ms > 0 else 0.0)
         sweep.append(row)
         log(json.dumps(row))
-    ix.set_timing(False)
 
     def best(target):
         ok = [r for r in sweep if r["recall"] >= target]
@@ -166,6 +172,7 @@ def main():
         except Exception as e:  # reported, never silently replaced
             log("learned leg failed: %r" % (e,))
 
+    ix.set_autotune(False)  # serving runs with the tiles already measured
     serve_out = None
     if not args.no_serve:
         try:
@@ -205,7 +212,7 @@ def main():
                      "launches_per_search": head["work"]["bands"],
                      "bytes_rule": "sum over touched lists of rows x (prefix bytes + 4-byte norm), once per search"},
         "stages_ms": head["stages_ms"],
-        "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs")}
+        "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs", "tile")}
                   for r in sweep],
         "learned_nprobe": learned_out,
         "serving": serve_out,

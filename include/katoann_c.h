@@ -183,8 +183,17 @@ int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant,
 /* Graph replay (no reference counterpart): each search context captures its
  * kernel sequence (normalisation through top-k) into a hipGraph once and
  * replays it while shapes, parameters and buffers stay the same (default on;
- * with stage timing on, the timing events are nodes of the graph). */
+ * searches with stage timing on run eagerly). */
 int katoann_set_graphs(katoann_index *h, int32_t on);
+/* Tile autotuning (no reference counterpart): with it on, the first search of
+ * each shape (ranking width, nlist, batch rounded up to a power of two,
+ * nprobe, max(survivors, k)) times the candidate ranking tiles (kernel, row
+ * tile, queries per item, band factor) on its own inputs and keeps the fastest
+ * for that shape; results are the same whatever the tile. Default off. */
+int katoann_set_autotune(katoann_index *h, int32_t on);
+/* The tile a search of this shape runs with: kernel, variant, queries per
+ * item, log2 band factor. */
+int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out4);
 
 /* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
  * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of

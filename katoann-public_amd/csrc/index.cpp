@@ -10,7 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <chrono>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -106,6 +109,13 @@ struct SearchCtx {
 
 }  // namespace
 
+// Tile shape of the ranking pass: kernel (0 block items, 1 wave items, 2 block
+// items with a query ring), row-tile variant, queries per item, log2 band factor.
+struct TileConfig {
+  uint32_t kernel, variant, qc, shift;
+};
+using TileKey = std::array<uint32_t, 6>;  // P, nlist, batch rounded up to 2^k, nprobe, S, prefix
+
 struct katoann_index {
   uint32_t dim = 0, nlist = 0, npts = 0, dc = 0, head_dims = 0;
   bool normalized = false, rotated = false, batched = true;
@@ -119,6 +129,9 @@ struct katoann_index {
   uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
   uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 512;
   std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
+  std::atomic<bool> autotune{false};
+  std::mutex tune_mu;
+  std::map<TileKey, TileConfig> tuned;  // per shape, the fastest tile measured
   uint64_t gen = 0;                // bumped by every load: captured graphs name the old buffers
   std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
   uint32_t tile_prefix_rows = 0;
@@ -135,6 +148,10 @@ struct katoann_index {
 
   void reset() {
     ++gen;
+    {
+      std::lock_guard<std::mutex> g(tune_mu);
+      tuned.clear();
+    }
     dim = nlist = npts = dc = head_dims = 0;
     normalized = rotated = false;
     batched = true;
@@ -423,8 +440,29 @@ struct SearchPlan {
   }
 };
 
+TileKey tile_key(const katoann_index &ix, uint32_t nq, const katoann_search_params &p) {
+  const bool prefix = p.ranking == KATOANN_RANK_PREFIX;
+  uint32_t b = 1;
+  while (b < nq) b <<= 1;
+  return TileKey{prefix ? ix.head_dims : ix.dc, ix.nlist, b, p.nprobe, std::max(p.survivors, p.k),
+                 prefix ? 1u : 0u};
+}
+
+// The tile a search runs with: the forced one, else the one measured fastest
+// for its shape, else the index's settings.
+TileConfig tile_for(const katoann_index &ix, uint32_t nq, const katoann_search_params &p, const TileConfig *force) {
+  if (force) return *force;
+  auto &mix = const_cast<katoann_index &>(ix);
+  {
+    std::lock_guard<std::mutex> g(mix.tune_mu);
+    auto it = mix.tuned.find(tile_key(ix, nq, p));
+    if (it != mix.tuned.end()) return it->second;
+  }
+  return TileConfig{ix.rank_kernel, ix.wave_variant, ix.wave_qc, ix.band_shift};
+}
+
 SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_search_params &p,
-                       const katoann_search_extras *x) {
+                       const katoann_search_extras *x, const TileConfig *force = nullptr) {
   if (ix.nlist == 0) throw logic_err("search before load");
   if (p.k == 0 || p.nprobe == 0 || p.nprobe > ix.nlist)
     throw std::invalid_argument("need k >= 1 and 1 <= nprobe <= nlist");
@@ -445,11 +483,12 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.P = z.prefix ? ix.head_dims : ix.dc;
   if (z.P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
   z.B = nq, z.d = ix.dim, z.dc = ix.dc, z.np = p.nprobe, z.K = p.k, z.nlist = ix.nlist;
-  z.shift = ix.band_shift;
-  z.wave = ix.rank_kernel >= 1;
-  z.ring = ix.rank_kernel == 2;
-  z.variant = ix.wave_variant;
-  z.qc = z.wave ? ix.wave_qc : katoann_hip::rank_queries_per_item(z.P);
+  const TileConfig tc = tile_for(ix, nq, p, force);
+  z.shift = tc.shift;
+  z.wave = tc.kernel >= 1;
+  z.ring = tc.kernel == 2;
+  z.variant = tc.variant;
+  z.qc = z.wave ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
   z.rt = z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1) : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
   z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift);
@@ -500,9 +539,10 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
 }
 
 void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
-                 const katoann_search_extras *x, uint32_t *out_ids, float *out_dists) {
+                 const katoann_search_extras *x, uint32_t *out_ids, float *out_dists,
+                 const TileConfig *force = nullptr) {
   if (nq == 0) return;
-  const SearchPlan z = plan_search(ix, nq, p, x);
+  const SearchPlan z = plan_search(ix, nq, p, x, force);
   const uint32_t B = z.B, d = z.d, dc = z.dc, np = z.np, K = z.K, nlist = z.nlist, S = z.S, P = z.P;
   const bool prefix = z.prefix;
   const bool dev_io = x && x->device_io;
@@ -652,7 +692,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       throw std::invalid_argument("unsupported refinement shape");
     mark(6);
   };
-  const bool graphs = mix.graphs.load();
+  const bool graphs = mix.graphs.load() && !timing;  // stage timing runs eagerly, events on the stream
   auto run = [&](uint64_t allowance) {
     const uint32_t stride = static_cast<uint32_t>(std::min<uint64_t>(S + allowance, 0xffffffffu));
     c.pool.ensure(std::max<uint64_t>(static_cast<uint64_t>(stride) * B, 1) * 8);
@@ -664,7 +704,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                    rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
-                                   x && x->survivors, x && x->n_survivors, x && x->codes, timing};
+                                   x && x->survivors, x && x->n_survivors, x && x->codes};
       for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
                               &c.npq, &c.gcnt, &c.qoff, &c.pscratch, &c.item_off, &c.qlist, &c.totals, &c.pool_cnt,
                               &c.pool_sorted, &c.thr, &c.lbound, &c.state, &c.pool, &c.surv, &c.nsurv, &c.codes,
@@ -864,9 +904,49 @@ int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t
   KATOANN_CATCH
 }
 
+namespace {
+
+// First search of a shape with autotuning on: every candidate tile runs twice
+// (the first run captures its graph), the second is timed; the fastest is kept
+// for the shape. Every tile gives the same results, so the last run's outputs
+// stand.
+void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
+                const katoann_search_extras *x, uint32_t *ids, float *dists) {
+  auto &mix = const_cast<katoann_index &>(ix);
+  if (!mix.autotune.load() || nq == 0 || ix.nlist == 0) return;
+  const TileKey key = tile_key(ix, nq, p);
+  {
+    std::lock_guard<std::mutex> g(mix.tune_mu);
+    if (mix.tuned.count(key)) return;
+  }
+  std::vector<TileConfig> cand;
+  for (uint32_t shift : {1u, 2u, 3u})
+    for (uint32_t variant : {0u, 1u})
+      for (uint32_t qc : {256u, 512u, 1024u}) cand.push_back(TileConfig{1u, variant, qc, shift});
+  for (uint32_t variant : {0u, 1u}) cand.push_back(TileConfig{2u, variant, 512u, 2u});
+  double best = 1e30;
+  TileConfig pick = cand[0];
+  for (const TileConfig &c : cand) {
+    if (katoann_hip::rank_wave_rows(key[0], c.variant) == 0) continue;
+    search_impl(ix, queries, nq, p, x, ids, dists, &c);
+    const auto t0 = std::chrono::steady_clock::now();
+    search_impl(ix, queries, nq, p, x, ids, dists, &c);
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best) {
+      best = dt;
+      pick = c;
+    }
+  }
+  std::lock_guard<std::mutex> g(mix.tune_mu);
+  mix.tuned[key] = pick;
+}
+
+}  // namespace
+
 int katoann_search(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
                    uint32_t *ids, float *dists) {
   KATOANN_TRY
+  maybe_tune(*h, queries, nq, *p, nullptr, ids, dists);
   search_impl(*h, queries, nq, *p, nullptr, ids, dists);
   KATOANN_CATCH
 }
@@ -878,7 +958,23 @@ int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq,
     for (uint32_t q = 0; q < nq; ++q)
       if (x->nprobe_per_query[q] == 0 || x->nprobe_per_query[q] > p->nprobe)
         throw std::invalid_argument("per-query nprobe must be in [1, nprobe]");
+  maybe_tune(*h, queries, nq, *p, x, ids, dists);
   search_impl(*h, queries, nq, *p, x, ids, dists);
+  KATOANN_CATCH
+}
+
+int katoann_set_autotune(katoann_index *h, int32_t on) {
+  h->autotune.store(on != 0);
+  return KATOANN_OK;
+}
+
+int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out) {
+  KATOANN_TRY
+  const TileConfig c = tile_for(*h, nq, *p, nullptr);
+  out[0] = c.kernel;
+  out[1] = c.variant;
+  out[2] = c.qc;
+  out[3] = c.shift;
   KATOANN_CATCH
 }
 

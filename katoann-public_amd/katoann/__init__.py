@@ -116,6 +116,8 @@ def lib():
         L.katoann_set_band_factor.argtypes = [P, u32]
         L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
         L.katoann_set_graphs.argtypes = [P, i32]
+        L.katoann_set_autotune.argtypes = [P, i32]
+        L.katoann_tile_config.argtypes = [P, u32, C.POINTER(_SearchParams), P]
         L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
         L.katoann_sq16_high_l2.argtypes = [P, P, P, P, f64, u32, u32, u32, P, u32]
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
@@ -376,6 +378,19 @@ class Index:
         items (rows held in registers, queries streamed); variant 0/1 = larger
         or smaller row tile per wave."""
         _check(lib().katoann_set_rank_kernel(self._h, kernel, variant, queries_per_item))
+
+    def set_autotune(self, on: bool = True):
+        """Time the candidate ranking tiles on the first search of each shape
+        and keep the fastest (results do not depend on the tile)."""
+        _check(lib().katoann_set_autotune(self._h, int(on)))
+
+    def tile_config(self, nq, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold"):
+        """The tile (kernel, variant, queries per item, log2 band factor) a
+        search of this shape runs with."""
+        out = np.zeros(4, np.uint32)
+        prm = self._params(k, nprobe, ranking, survivors, selection)
+        _check(lib().katoann_tile_config(self._h, nq, C.byref(prm), _ptr(out)))
+        return dict(kernel=int(out[0]), variant=int(out[1]), queries_per_item=int(out[2]), log2_band_factor=int(out[3]))
 
     def set_graphs(self, on: bool = True):
         """Replay each search's kernels from a captured hipGraph (default on)."""

@@ -244,3 +244,21 @@ def test_graph_replays_match_eager(mixture_index):
         ix.set_graphs(False)
         i3, _, _ = ix.search_ex(q[:300], k=10, nprobe=16, normalize=False)
         ix.set_graphs(True)
+
+
+def test_autotune_keeps_results(mixture_index):
+    """With autotuning on, the first search of a shape times the candidate
+    tiles and keeps one; every later search of that shape runs with it, and
+    results equal those of the default tile."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ref = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)[:2]
+    default = ix.tile_config(q.shape[0], k=10, nprobe=64, survivors=64)
+    ix.set_autotune(True)
+    for rep in range(2):
+        print("autotune", rep, flush=True)
+        i, d, _ = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)
+        assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
+    tuned = ix.tile_config(q.shape[0], k=10, nprobe=64, survivors=64)
+    assert tuned["kernel"] in (1, 2) and tuned["queries_per_item"] in (256, 512, 1024)
+    print("default tile", default, "tuned", tuned, flush=True)
