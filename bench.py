@@ -50,12 +50,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="spectrum", choices=("spectrum", "zipf"),
+                    help="spectrum: 1M x 1024 i^-1 spectrum, nlist 4096 (default); zipf: 10M x 1024 mixture of "
+                         "16,384 components with Zipf(1.1) weights, nlist 16,384")
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--dim", type=int, default=1024)
     ap.add_argument("--nq", type=int, default=4096)
-    ap.add_argument("--nlist", type=int, default=4096)
+    ap.add_argument("--nlist", type=int, default=None)
     ap.add_argument("--alpha", type=float, default=1.0)
-    ap.add_argument("--nprobe", default="16,32,64,128,256,512,1024,2048")
+    ap.add_argument("--nprobe", default=None)
     ap.add_argument("--survivors", default="32,128,256")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--kmeans-iters", type=int, default=10)
@@ -71,6 +74,16 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
     args = ap.parse_args()
+    zipf = args.workload == "zipf"
+    if args.n is None:
+        args.n = 10_000_000 if zipf else 1_000_000
+    if args.nlist is None:
+        args.nlist = 16384 if zipf else 4096
+    if args.nprobe is None:
+        args.nprobe = "32,64,128,256,512" if zipf else "16,32,64,128,256,512,1024,2048"
+    if zipf:  # the learned leg's workload and the reference CPU run (41 GB of files) are not this one's
+        args.no_learned = True
+        args.no_cpu_baseline = True
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (--gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -82,7 +95,10 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     t0 = time.time()
-    base, queries = datasets.spectrum_torch(args.n, args.nq, args.dim, alpha=args.alpha, seed=2, device=dev)
+    if zipf:
+        base, queries = datasets.zipf_mixture_torch(args.n, args.nq, args.dim, comps=16384, s=1.1, seed=3, device=dev)
+    else:
+        base, queries = datasets.spectrum_torch(args.n, args.nq, args.dim, alpha=args.alpha, seed=2, device=dev)
     torch.cuda.synchronize()
     log("data %.1fs" % (time.time() - t0))
     t0 = time.time()
@@ -100,6 +116,8 @@ def main():
                    order.cpu().numpy().astype(np.uint32), None, b["rotation"].cpu().numpy(), pd,
                    vectors_device_ptr=vecs_sorted.data_ptr())
     torch.cuda.synchronize()
+    del vecs_sorted
+    torch.cuda.empty_cache()
     log("load (GPU rotation + quantisation) %.1fs, list sizes min %d median %d max %d" % (
         time.time() - t0, sizes.min(), int(np.median(sizes)), sizes.max()))
     if args.rank_rows:
@@ -149,12 +167,15 @@ def main():
                                 want=("probes",))
         touched = np.unique(ex["probes"])
         rank_bytes = float(sizes[touched].astype(np.float64).sum() * (pd + 4))
+        # algorithmic int8 work of the ranking pass: 2 P ops per (query, candidate row)
+        rank_ops = float(sizes[ex["probes"]].astype(np.float64).sum() * 2 * pd)
         stages = dict(zip(katoann.STAGES, (stage_sum / args.steps).tolist()))
         work = ix.last_work()
         rank_ms = stages["rank"]
         row = dict(nprobe=np_, survivors=S, ms_per_step=ms, qps=nq / (ms / 1e3), recall=rec, stages_ms=stages,
                    tile=ix.tile_config(nq, k=k, nprobe=np_, survivors=S),
-                   work=work, rank_bytes=rank_bytes, rank_gbs=rank_bytes / (rank_ms * 1e-3) / 1e9 if rank_ms > 0 else 0.0)
+                   work=work, rank_bytes=rank_bytes, rank_gbs=rank_bytes / (rank_ms * 1e-3) / 1e9 if rank_ms > 0 else 0.0,
+                   rank_ops=rank_ops, rank_tops=rank_ops / (rank_ms * 1e-3) / 1e12 if rank_ms > 0 else 0.0)
         sweep.append(row)
         log(json.dumps(row))
 
@@ -200,25 +221,44 @@ def main():
         "vs_baseline": None,
         "dtype": "u8/int32 (16-bit codes, exact integer distances)",
         "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": "%dx%d i^-%g spectrum, nlist %d, prefix %d, k %d, batch %d" % (
-            args.n, args.dim, args.alpha, args.nlist, pd, k, nq),
+        "config": {"workload": ("%dx%d mixture of 16384 components, Zipf(1.1) weights, i^-1 spectrum" % (
+            args.n, args.dim) if zipf else "%dx%d i^-%g spectrum" % (args.n, args.dim, args.alpha)) +
+            ", nlist %d, prefix %d, k %d, batch %d" % (args.nlist, pd, k, nq),
             "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"]},
         "qps_at_0999": b999["qps"] if b999 else None,
         "at_0999": {"nprobe": b999["nprobe"], "survivors": b999["survivors"]} if b999 else None,
-        "roofline": {"bound": "hbm", "kernel": "k_rank_band (the probe-rank bands of one search)",
-                     "achieved": head["rank_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["rank_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_search": head["rank_bytes"], "ms_per_search": head["stages_ms"]["rank"],
-                     "launches_per_search": head["work"]["bands"],
-                     "bytes_rule": "sum over touched lists of rows x (prefix bytes + 4-byte norm), once per search"},
+        "roofline": roofline(head),
         "stages_ms": head["stages_ms"],
-        "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs", "tile")}
+        "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs", "rank_tops", "tile")}
                   for r in sweep],
         "learned_nprobe": learned_out,
         "serving": serve_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+MFMA_I8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
+
+
+def roofline(r):
+    """Roofline of the ranking pass (k_rank_wave, all probe-rank bands of one
+    search): algorithmic bytes = rows of the touched lists x (P + 4), once per
+    search; algorithmic int8 ops = 2 P per (query, candidate row). The bound is
+    the one its arithmetic intensity puts it under (ridge 5000 TOP/s / 8000
+    GB/s = 625 ops per byte)."""
+    inten = r["rank_ops"] / r["rank_bytes"] if r["rank_bytes"] else 0.0
+    mfma = inten > MFMA_I8_PEAK_TOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    ach, peak, unit = (r["rank_tops"], MFMA_I8_PEAK_TOPS, "TOP/s") if mfma else (r["rank_gbs"], HBM_PEAK_GBS, "GB/s")
+    return {"bound": "mfma" if mfma else "hbm", "kernel": "k_rank_wave (all probe-rank bands of one search)",
+            "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
+            "intensity_ops_per_byte": inten, "hbm_gbs": r["rank_gbs"], "hbm_frac": r["rank_gbs"] / HBM_PEAK_GBS,
+            "int8_tops": r["rank_tops"], "int8_frac": r["rank_tops"] / MFMA_I8_PEAK_TOPS,
+            "bytes_per_search": r["rank_bytes"], "ops_per_search": r["rank_ops"],
+            "ms_per_search": r["stages_ms"]["rank"], "launches_per_search": r["work"]["bands"],
+            "bytes_rule": "sum over touched lists of rows x (prefix bytes + 4-byte norm), once per search",
+            "ops_rule": "2 x prefix bytes per (query, row of a probed list)",
+            "timing": "HIP events on the search stream around every ranking launch, averaged over the steps"}
 
 
 def learned_leg(ix, base, queries, gt, b, sizes, args, plain, ids_d, dists_d):
