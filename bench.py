@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the sweep, copy this many MiB device to device 3 times (a plain copy kernel of "
+                         "known traffic, for scaling profiler byte counters)")
     args = ap.parse_args()
     zipf = args.workload == "zipf"
     if args.n is None:
@@ -179,6 +182,15 @@ def main():
         sweep.append(row)
         log(json.dumps(row))
 
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     def best(target):
         ok = [r for r in sweep if r["recall"] >= target]
         return max(ok, key=lambda r: r["qps"]) if ok else None
@@ -227,7 +239,7 @@ def main():
             "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"]},
         "qps_at_0999": b999["qps"] if b999 else None,
         "at_0999": {"nprobe": b999["nprobe"], "survivors": b999["survivors"]} if b999 else None,
-        "roofline": roofline(head),
+        "roofline": dict(roofline(head), traffic=profiled_traffic(head, args)),
         "stages_ms": head["stages_ms"],
         "sweep": [{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs", "rank_tops", "tile")}
                   for r in sweep],
@@ -239,6 +251,21 @@ def main():
 
 
 MFMA_I8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
+
+
+def profiled_traffic(r, args):
+    """HBM bytes per search of the ranking pass from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, when it was taken on this configuration."""
+    path = os.path.join(ROOT, "profiles", "k_rank_wave_1Mx1024_nlist4096_b4096_np1024_S128.md")
+    want = "spectrum %dx%d nlist %d nprobe %d survivors %d batch %d" % (
+        args.n, args.dim, args.nlist, r["nprobe"], r["survivors"], args.nq)
+    try:
+        lines = dict(l.split(": ", 1) for l in open(path).read().splitlines() if l.startswith("traffic_"))
+    except OSError:
+        return None
+    if args.workload != "spectrum" or lines.get("traffic_config", "").strip() != want:
+        return None
+    return float(lines["traffic_bytes_per_search"])
 
 
 def roofline(r):
