@@ -525,6 +525,200 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
   }
 }
 
+// Probe selection with the coarse row in registers (nlist <= 256 V): thread t
+// owns lists [t V, t V + V). The np-th smallest value is found by radix select
+// on range-relative float bits (per-wave LDS histograms), the kept keys are
+// collected in list order through block scans (values below the pivot, then
+// the lowest-numbered lists at the pivot), each wave sorts a quarter of them
+// in registers, and two merge-path levels finish the sort.
+__device__ inline uint32_t block_exscan256(uint32_t v, uint32_t *tmp) {
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[w] = x;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int k = 0; k < w; ++k) base += tmp[k];
+  __syncthreads();
+  return base + x - v;
+}
+
+template <int W>
+__device__ inline void wave_sort_run(uint64_t *run) {
+  const int lane = lane_id();
+  TopList<W> t;
+#pragma unroll
+  for (int r = 0; r < W / 64; ++r) t.e[r] = run[64 * r + lane];
+  list_sort<W>(t);
+#pragma unroll
+  for (int r = 0; r < W / 64; ++r) run[64 * r + lane] = t.e[r];
+}
+
+__device__ inline void sort_runs(uint64_t *buf, uint32_t W, uint32_t nruns) {
+  const int w = threadIdx.x >> 6;
+  if (static_cast<uint32_t>(w) >= nruns) return;
+  uint64_t *run = buf + static_cast<size_t>(w) * W;
+  switch (W) {
+    case 64: wave_sort_run<64>(run); break;
+    case 128: wave_sort_run<128>(run); break;
+    case 256: wave_sort_run<256>(run); break;
+    case 512: wave_sort_run<512>(run); break;
+    default: wave_sort_run<1024>(run); break;
+  }
+}
+
+// number of keys in sorted run a[0, n) below x (upper: at or below x)
+__device__ __forceinline__ uint32_t count_below(const uint64_t *a, uint32_t n, uint64_t x, bool upper) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (upper ? a[mid] <= x : a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
+                                                    uint32_t np, uint32_t *__restrict__ probes,
+                                                    float *__restrict__ probe_d) {
+  extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x M
+  __shared__ uint32_t hist[4][SEL_BINS];
+  __shared__ uint32_t tmp[8], sh_digit, sh_rank;
+  const uint32_t q = blockIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  uint32_t M = 64;
+  while (M < np) M <<= 1;
+  const uint32_t *row = reinterpret_cast<const uint32_t *>(dist) + static_cast<size_t>(q) * nlist;
+  const uint32_t c0 = static_cast<uint32_t>(tid) * V;
+  uint32_t v[V];
+  if ((nlist & 3u) == 0 && c0 + V <= nlist) {
+#pragma unroll
+    for (int i = 0; i < V; i += 4) {
+      const uint4 x = *reinterpret_cast<const uint4 *>(row + c0 + i);
+      v[i] = x.x;
+      v[i + 1] = x.y;
+      v[i + 2] = x.z;
+      v[i + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = c0 + i < nlist ? row[c0 + i] : 0xffffffffu;
+  }
+  // range of the row (lists past nlist hold 0xffffffff, above every distance)
+  uint32_t vmin = 0xffffffffu, vmax = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    if (c0 + i < nlist) {
+      vmin = min(vmin, v[i]);
+      vmax = max(vmax, v[i]);
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), o)));
+    vmax = max(vmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmax), o)));
+  }
+  if (lane == 0) {
+    tmp[w] = vmin;
+    tmp[4 + w] = vmax;
+  }
+  __syncthreads();
+  vmin = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
+  vmax = max(max(tmp[4], tmp[5]), max(tmp[6], tmp[7]));
+  __syncthreads();
+  // radix select of the np-th smallest on r = v - vmin, 11-bit digits
+  const uint32_t range = vmax - vmin;
+  const int nbits = range ? 32 - __clz(range) : 0;
+  uint32_t prefix = 0, mask = 0, rank = np;
+  for (int top = nbits; top > 0; top -= 11) {
+    const int shift = max(0, top - 11);
+    const uint32_t dmask = (1u << (top - shift)) - 1u;
+    for (int i = tid; i < 4 * SEL_BINS; i += 256) (&hist[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const uint32_t r = v[i] - vmin;
+      if (c0 + i < nlist && (r & mask) == prefix) atomicAdd(&hist[w][(r >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    // bins 8 tid .. 8 tid + 7, summed over the waves; the bin holding the rank-th
+    uint32_t h[8], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int b = 8 * tid + i;
+      h[i] = hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
+      sum += h[i];
+    }
+    uint32_t cum = block_exscan256(sum, tmp);
+    if (cum < rank && rank <= cum + sum) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (cum < rank && rank <= cum + h[i]) {
+          sh_digit = 8 * tid + i;
+          sh_rank = rank - cum;
+        }
+        cum += h[i];
+      }
+    }
+    __syncthreads();
+    prefix |= sh_digit << shift;
+    mask |= dmask << shift;
+    rank = sh_rank;
+    __syncthreads();
+  }
+  // the kept keys in list order: below the pivot, then the first need_eq at it
+  const uint32_t T = vmin + prefix, need_eq = rank, n_lt = np - need_eq;
+  uint32_t clt = 0, ceq = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    if (c0 + i < nlist) {
+      clt += v[i] < T ? 1u : 0u;
+      ceq += v[i] == T ? 1u : 0u;
+    }
+  uint32_t olt = block_exscan256(clt, tmp);
+  uint32_t oeq = block_exscan256(ceq, tmp);
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    if (c0 + i < nlist) {
+      const uint64_t key = (static_cast<uint64_t>(v[i]) << 32) | (c0 + i);
+      if (v[i] < T) keys[olt++] = key;
+      else if (v[i] == T) {
+        if (oeq < need_eq) keys[n_lt + oeq] = key;
+        ++oeq;
+      }
+    }
+  for (uint32_t p = np + tid; p < M; p += 256) keys[p] = kNoKey;
+  __syncthreads();
+  // sort: runs of W keys by the waves, then merge levels
+  const uint32_t W = M >= 256 ? M / 4 : M, nruns = M / W;
+  sort_runs(keys, W, nruns);
+  __syncthreads();
+  uint64_t *src = keys, *dst = keys + M;
+  for (uint32_t L = W; L < M; L <<= 1) {
+    for (uint32_t e = tid; e < M; e += 256) {
+      const uint32_t pair = e / (2 * L) * (2 * L), off = e - pair;
+      const uint64_t x = src[e];
+      const bool inA = off < L;
+      const uint64_t *other = src + pair + (inA ? L : 0);
+      const uint32_t pos = (inA ? off : off - L) + count_below(other, L, x, !inA);
+      dst[pair + pos] = x;
+    }
+    __syncthreads();
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  for (uint32_t p = tid; p < np; p += 256) {
+    const uint64_t k = src[p];
+    probes[static_cast<size_t>(q) * np + p] = static_cast<uint32_t>(k);
+    probe_d[static_cast<size_t>(q) * np + p] = __uint_as_float(static_cast<uint32_t>(k >> 32));
+  }
+}
+
 // --------------------------------------------------------------------------
 // Inversion (src/index.cpp:403-412) and the work plan, list-major and without
 // global atomics. Work is grouped by (probe-rank band, list): for band factor
@@ -842,8 +1036,25 @@ size_t select_smem(uint32_t nlist, uint32_t np) {
   return (nlist <= SEL_CACHE ? static_cast<size_t>(nlist) * 4 : 0) + static_cast<size_t>(M) * 8;
 }
 
+template <int V>
+static void select_reg_launch(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes,
+                              float *probe_d, hipStream_t s) {
+  uint32_t M = 64;
+  while (M < np) M <<= 1;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_select_reg<V>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(2 * SEL_MAXNP * 8));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_select_reg<V>, dim3(B), dim3(256), 2 * M * 8, s, dist, B, nlist, np, probes, probe_d);
+}
+
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
                    hipStream_t s) {
+  if (nlist <= 256 * 16) return select_reg_launch<16>(dist, B, nlist, np, probes, probe_d, s);
+  if (nlist <= 256 * 32) return select_reg_launch<32>(dist, B, nlist, np, probes, probe_d, s);
+  if (nlist <= 256 * 64) return select_reg_launch<64>(dist, B, nlist, np, probes, probe_d, s);
   const size_t sm = select_smem(nlist, np);
   static bool attr = false;
   if (!attr) {

@@ -262,3 +262,29 @@ def test_autotune_keeps_results(mixture_index):
     tuned = ix.tile_config(q.shape[0], k=10, nprobe=64, survivors=64)
     assert tuned["kernel"] in (1, 2) and tuned["queries_per_item"] in (256, 512, 1024)
     print("default tile", default, "tuned", tuned, flush=True)
+
+
+@pytest.fixture(scope="module")
+def many_lists(tmp_path_factory):
+    """Coarse rows of 5,000 and 16,384 lists (the selection's register rows of
+    32 and 64 lists per thread), built on the GPU, written as files."""
+    import torch
+    from katoann import builder, datasets
+    d = tmp_path_factory.mktemp("many")
+    base, q = datasets.spectrum_torch(40000, 200, 128, alpha=1.0, seed=9, device="cuda")
+    out = {}
+    for nl in (5000, 16384):
+        prefix = str(d / ("nl%d" % nl))
+        builder.build_torch(base, prefix, nlist=nl, train_size=40000, kmeans_iters=3, seed=1, normalized=True)
+        out[nl] = prefix
+    return out, q.cpu().numpy()
+
+
+@pytest.mark.parametrize("nl", [5000, 16384])
+def test_selection_many_lists_vs_oracle(many_lists, nl):
+    prefixes, q = many_lists
+    ix = load(prefixes[nl])
+    ox = _oracle(prefixes[nl])
+    for nprobe in (1, 77, 4096):
+        print("many lists", nl, nprobe, flush=True)
+        check_against_oracle(ix, ox, q[:64], 10, nprobe, 32)
