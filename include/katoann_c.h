@@ -171,7 +171,7 @@ int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n);
 /* Tile parameters of the ranking pass: rows per work item (multiple of 64). */
 int katoann_set_rank_rows(katoann_index *h, uint32_t rows);
 /* Probe-rank bands of the ranking pass: band b holds ranks [(F^b - 1) / (F - 1),
- * (F^(b+1) - 1) / (F - 1)) with F = 2^log2_factor (1..4); the queries' thresholds
+ * (F^(b+1) - 1) / (F - 1)) with F = 2^log2_factor (1..12); the queries' thresholds
  * tighten between bands. Default 2 (F = 4). */
 int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
 /* Ranking kernel: 0 = block items (a list's queries staged into LDS, rows

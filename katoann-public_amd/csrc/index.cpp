@@ -35,6 +35,7 @@ constexpr uint32_t kDimcMagic = 0x434D4944u;  // index.cpp:27
 constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
 constexpr uint32_t kMaxSurvivors = 1024;
 constexpr uint32_t kPoolAllowance = 1024;  // pool slots per query beyond S for one band's kept keys
+constexpr uint64_t kPoolAllowanceCap = 65536;  // ... at most, unless one list is larger (overflow reruns)
 constexpr uint32_t kMaxProbes = 4096;
 constexpr uint32_t kMaxBands = 16;  // probe-rank bands at most (nprobe 4096, factor 2: 13)
 
@@ -506,8 +507,10 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
   // the first bands run without a threshold (or with one from few candidates):
   // room for every row of a query's 1 + F nearest lists
-  z.allowance = std::min(z.max_rows, std::max<uint64_t>(kPoolAllowance,
-                                                        mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift)))));
+  z.allowance = std::min(
+      z.max_rows,
+      std::max<uint64_t>({kPoolAllowance, 2 * mix.max_rows_per_query(1),
+                          std::min(kPoolAllowanceCap, mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift))))}));
   const size_t B = z.B;
   z.q = B * z.d * 4;
   z.qcodes = B * z.dc;
@@ -747,10 +750,14 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     HIPCHECK(hipStreamSynchronize(s));
     return st[0] == 0;
   };
-  const uint64_t allowance = z.allowance;
-  if (!run(allowance) && (allowance >= max_rows || !run(max_rows))) {
-    ix.release(std::move(ctx));
-    throw std::runtime_error("survivor pool overflow in the ranking pass");
+  // on a pool overflow the batch runs again with 4x the room, up to every candidate
+  uint64_t allowance = z.allowance;
+  while (!run(allowance)) {
+    if (allowance >= max_rows) {
+      ix.release(std::move(ctx));
+      throw std::runtime_error("survivor pool overflow in the ranking pass");
+    }
+    allowance = std::min<uint64_t>(max_rows, allowance * 4);
   }
 
   const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -919,15 +926,10 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
     std::lock_guard<std::mutex> g(mix.tune_mu);
     if (mix.tuned.count(key)) return;
   }
-  std::vector<TileConfig> cand;
-  for (uint32_t shift : {1u, 2u, 3u})
-    for (uint32_t variant : {0u, 1u})
-      for (uint32_t qc : {256u, 512u, 1024u}) cand.push_back(TileConfig{1u, variant, qc, shift});
-  for (uint32_t variant : {0u, 1u}) cand.push_back(TileConfig{2u, variant, 512u, 2u});
   double best = 1e30;
-  TileConfig pick = cand[0];
-  for (const TileConfig &c : cand) {
-    if (katoann_hip::rank_wave_rows(key[0], c.variant) == 0) continue;
+  TileConfig pick{1u, 0u, 1024u, 3u};
+  auto trial = [&](const TileConfig &c) {
+    if (katoann_hip::rank_wave_rows(key[0], c.variant) == 0) return;
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
     const auto t0 = std::chrono::steady_clock::now();
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
@@ -936,7 +938,19 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
       best = dt;
       pick = c;
     }
+  };
+  // the band factor first (distinct band counts only), then the tile
+  uint32_t last_nb = 0;
+  for (uint32_t shift : {1u, 2u, 3u, 4u, 5u, 6u, 8u, 10u, 12u}) {
+    const uint32_t nb = katoann_hip::plan_bands(p.nprobe, shift);
+    if (nb == last_nb) continue;
+    last_nb = nb;
+    trial(TileConfig{1u, 0u, 1024u, shift});
   }
+  const uint32_t shift = pick.shift;
+  for (uint32_t variant : {0u, 1u})
+    for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift});
+  for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift});
   std::lock_guard<std::mutex> g(mix.tune_mu);
   mix.tuned[key] = pick;
 }
@@ -1072,7 +1086,7 @@ int katoann_set_rank_rows(katoann_index *h, uint32_t rows) {
 
 int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
   KATOANN_TRY
-  if (log2_factor < 1 || log2_factor > 4) throw std::invalid_argument("band factor must be 2, 4, 8 or 16");
+  if (log2_factor < 1 || log2_factor > 12) throw std::invalid_argument("band factor must be 2^1 .. 2^12");
   std::lock_guard<std::mutex> g(h->mu);
   h->band_shift = log2_factor;
   KATOANN_CATCH

@@ -480,12 +480,11 @@ __global__ __launch_bounds__(256, 2) void k_rank_band(BandArgs a) {
 // two fragments ahead, codes, norms and bounds one fragment ahead. No LDS and
 // no block barriers: the waves of a CU overlap one another's row loads, MFMAs
 // and filters freely.
-//   Per row x: a = floor(xn / 2); per query: h = qn - L.
-//   Keep  <=>  qn + xn - 2 dot <= L  <=>  2 dot - xn >= h, which needs
-//   dot - a >= floor(h / 2): the hot test is one add per row, a max over the
-//   rows and one compare per query. Fragments that pass it take the exact
-//   test and form the key f(qn + xn - 2 dot).
-// Kept keys go to the pools through the wave's deferred stage (keep_fragment).
+//   Keep  <=>  qn + xn - 2 dot <= L  <=>  2 dot - xn >= qn - L. The MFMA
+//   accumulators start at -ceil(xn / 2), so the test of a fragment is a max
+//   over the lane's rows and one compare per query slot; fragments in which a
+//   row passes form the keys f(qn + xn - 2 dot) (keep_exact).
+// Kept keys go to the pools through the wave's deferred stage.
 // Items are dealt to the waves round-robin (no shared counter).
 struct WaveArgs {
   const int8_t *plane;       // N x P, row-major, shifted high bytes
@@ -543,82 +542,113 @@ __device__ __forceinline__ void keep_drain(KeepStage &ks, const WaveArgs &a) {
   ks.pending = 0;
 }
 
-// Exact test of a fragment that passed the hot test, and the appends of its
-// keys. acc[t][r]: dot of row 16 t + c16 with query slot 4 g16 + r; h4[r]:
-// qn - L of that query (INT32_MAX past the item); qslot(j) / qnorm_slot(j): id
-// and norm of query slot j (lane-uniform calls). Returns the keys kept.
+// Inclusive prefix sums within each 16-lane row of the wave (DPP row shifts).
+__device__ __forceinline__ uint32_t row_scan16(uint32_t x) {
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x111, 0xf, 0xf, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x112, 0xf, 0xf, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x114, 0xf, 0xf, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x118, 0xf, 0xf, true));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t byte_sum(uint32_t x) {
+  return (x & 0xffu) + ((x >> 8) & 0xffu) + ((x >> 16) & 0xffu) + (x >> 24);
+}
+
+// v[t] for a lane-varying t < T.
+template <int T, class V>
+__device__ __forceinline__ V pick(const V (&v)[T], int t) {
+  V r = v[0];
+#pragma unroll
+  for (int i = 1; i < T; ++i) {
+    r = t == i ? v[i] : r;
+    asm volatile("" : "+v"(r));  // a select chain, not an indexed stack array
+  }
+  return r;
+}
+
+// The filter of a fragment. The accumulators are seeded with -ceil(xn / 2), so
+// acc[t][r] = dot(row 16 t + c16, query slot 4 g16 + r) - ceil(xn / 2), and a
+// row is kept iff acc >= H with H = floor((qn - L) / 2), i.e. iff
+// 2 dot - xn >= 2 floor((qn - L) / 2): every row with l2 <= L, plus those with
+// l2 = L + 1 when qn - L is odd. The extra rows are real candidates of the
+// query, and a pool keeps its S smallest keys, so they change no result.
+// keep_exact takes a fragment in which some row passed: the lane's kept rows
+// as bit masks per query slot, their runs in the wave's stage (row-wise prefix
+// sums of byte-packed counts), and the keys f(qn - 2 acc - (xn & 1)). H4[r]:
+// H of slot 4 g16 + r; vmask / oddm: bit t set iff row 16 t + c16 is in the
+// item / has an odd norm; qslot(j) / qnorm_slot(j): id and norm of query slot j.
+// Returns the keys kept.
 template <int T, class QS, class QN>
-__device__ __forceinline__ uint32_t keep_fragment(KeepStage &ks, const WaveArgs &a, v4i (&acc)[T],
-                                                  const int32_t (&xn)[T], const uint32_t (&xid)[T], uint32_t rows,
-                                                  const int32_t (&h4)[4], QS qslot, QN qnorm_slot) {
+__device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a, const v4i (&acc)[T],
+                                               uint32_t vmask, uint32_t oddm, const uint32_t (&xid)[T],
+                                               const int32_t (&H4)[4], QS qslot, QN qnorm_slot) {
   const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
-  const uint64_t gm = 0xffffull << (16 * g16);
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
-  auto keep = [&](int t, int r) { return 16u * t + c16 < rows && 2 * acc[t][r] - xn[t] >= h4[r]; };
-  uint32_t cnt[4];
+  uint32_t bits[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    cnt[r] = 0;
+    uint32_t m = 0;
+    // bit t <- sign of H - 1 - acc (set iff acc >= H), shifted in from t = T - 1 down
 #pragma unroll
-    for (int t = 0; t < T; ++t) cnt[r] += static_cast<uint32_t>(__popcll(ballot(keep(t, r)) & gm));
+    for (int t = T - 1; t >= 0; --t)
+      m = __builtin_amdgcn_alignbit(m, static_cast<uint32_t>(H4[r] - 1 - acc[t][r]), 31);
+    bits[r] = m & vmask;
   }
-  // slot j = 4 g16 + r: run offsets in the stage (exclusive prefix over j)
-  const uint32_t G = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-  const uint32_t G0 = __shfl(G, 0), G1 = __shfl(G, 16), G2 = __shfl(G, 32), G3 = __shfl(G, 48);
-  const uint32_t total = G0 + G1 + G2 + G3;
-  uint32_t so[4];
-  so[0] = (g16 > 0 ? G0 : 0u) + (g16 > 1 ? G1 : 0u) + (g16 > 2 ? G2 : 0u);
+  const uint32_t c = static_cast<uint32_t>(__builtin_popcount(bits[0])) |
+                     static_cast<uint32_t>(__builtin_popcount(bits[1])) << 8 |
+                     static_cast<uint32_t>(__builtin_popcount(bits[2])) << 16 |
+                     static_cast<uint32_t>(__builtin_popcount(bits[3])) << 24;
+  // per 16-lane group g, byte r: keys of slot 4 g + r (at most 16 T < 256)
+  const uint32_t inc = row_scan16(c), ex = inc - c;
+  const uint32_t G0 = __builtin_amdgcn_readlane(inc, 15), G1 = __builtin_amdgcn_readlane(inc, 31);
+  const uint32_t G2 = __builtin_amdgcn_readlane(inc, 47), G3 = __builtin_amdgcn_readlane(inc, 63);
+  const uint32_t S0 = byte_sum(G0), S1 = byte_sum(G1), S2 = byte_sum(G2), S3 = byte_sum(G3);
+  const uint32_t total = S0 + S1 + S2 + S3;
+  if (total == 0) return 0;
+  const uint32_t Gm = g16 == 0 ? G0 : g16 == 1 ? G1 : g16 == 2 ? G2 : G3;
+  uint32_t so[4];  // run of slot 4 g16 + r in the stage
+  so[0] = (g16 > 0 ? S0 : 0u) + (g16 > 1 ? S1 : 0u) + (g16 > 2 ? S2 : 0u);
 #pragma unroll
-  for (int r = 1; r < 4; ++r) so[r] = so[r - 1] + cnt[r - 1];
-  const uint32_t mine = c16 == 0 ? cnt[0] : c16 == 1 ? cnt[1] : c16 == 2 ? cnt[2] : c16 == 3 ? cnt[3] : 0u;
+  for (int r = 1; r < 4; ++r) so[r] = so[r - 1] + ((Gm >> (8 * (r - 1))) & 0xffu);
+  const uint32_t mine = c16 < 4 ? (Gm >> (8 * c16)) & 0xffu : 0u;
   const uint32_t qmine = qslot(4 * g16 + (c16 & 3));
-  // the tests again rather than 4 T masks kept live from the count
-#pragma unroll
-  for (int t = 0; t < T; ++t) asm volatile("" : "+v"(acc[t]));
-  if (total > static_cast<uint32_t>(KCAP)) {
+  // a fragment with more than KCAP keys (the first band keeps every candidate)
+  // is written to the pools at once
+  const bool direct = total > static_cast<uint32_t>(KCAP);
+  uint32_t base = 0;
+  if (direct) {
     keep_drain(ks, a);
-    uint32_t base = 0;
     if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int j = 4 * g16 + r;
-      uint32_t off = __shfl(base, 16 * g16 + r);
-      const uint32_t q = qslot(j);
-      const int32_t qnr = qnorm_slot(j);
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const bool bit = keep(t, r);
-        const uint64_t m = ballot(bit) & gm;
-        if (bit) {
-          const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-          const uint64_t key = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
-          if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
-          else atomicOr(&a.state[0], 1u);
-        }
-        off += static_cast<uint32_t>(__popcll(m));
-      }
-    }
-    return total;
   }
   const int reg = ks.par ^ 1;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int j = 4 * g16 + r;
-    uint32_t off = so[r];
     const int32_t qnr = qnorm_slot(j);
+    const uint32_t q = qslot(j);
+    const uint32_t boff = __shfl(base, 16 * g16 + r);
+    uint32_t idx = (ex >> (8 * r)) & 0xffu;  // the lane's first key within the slot's run
+    int32_t av[T];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const bool bit = keep(t, r);
-      const uint64_t m = ballot(bit) & gm;
-      if (bit) {
-        const uint32_t pos = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-        ks.key[reg * KCAP + pos] = make_key(fscale(qnr + xn[t] - 2 * acc[t][r], a.step_sq), xid[t]);
-        ks.meta[reg * KCAP + pos] = static_cast<uint32_t>(j) | ((pos - so[r]) << 4);
+    for (int t = 0; t < T; ++t) av[t] = acc[t][r];
+    uint32_t m = bits[r];
+    while (m) {
+      const int t = __builtin_ctz(m);
+      m &= m - 1u;
+      const int32_t l2 = qnr - 2 * pick(av, t) - static_cast<int32_t>((oddm >> t) & 1u);
+      const uint64_t key = make_key(fscale(l2, a.step_sq), pick(xid, t));
+      if (direct) {
+        const uint32_t pos = boff + idx;
+        if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+        else atomicOr(&a.state[0], 1u);
+      } else {
+        ks.key[reg * KCAP + so[r] + idx] = key;
+        ks.meta[reg * KCAP + so[r] + idx] = static_cast<uint32_t>(j) | (idx << 4);
       }
-      off += static_cast<uint32_t>(__popcll(m));
+      ++idx;
     }
   }
-  uint32_t base = 0;
+  if (direct) return total;
   if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
   keep_drain(ks, a);
   ks.par = reg;
@@ -628,6 +658,36 @@ __device__ __forceinline__ uint32_t keep_fragment(KeepStage &ks, const WaveArgs 
   return total;
 }
 
+// Per item: the MFMA seeds -ceil(xn / 2) of the lane's rows, and the masks of
+// rows in the item and of odd norms.
+template <int T>
+__device__ __forceinline__ void row_seeds(const int32_t (&xn)[T], uint32_t rows, v4i (&seed)[T], uint32_t &vmask,
+                                          uint32_t &oddm) {
+  const int c16 = lane_id() & 15;
+  vmask = 0;
+  oddm = 0;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int32_t c = -((xn[t] + 1) >> 1);
+    seed[t] = v4i{c, c, c, c};
+    vmask |= static_cast<uint32_t>(16u * t + c16 < rows) << t;
+    oddm |= static_cast<uint32_t>(xn[t] & 1) << t;
+  }
+}
+
+// Whether any row of the fragment passes: max over the lane's rows per slot.
+template <int T>
+__device__ __forceinline__ bool fragment_passes(const v4i (&acc)[T], const int32_t (&H4)[4]) {
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int32_t m = acc[0][r];
+#pragma unroll
+    for (int t = 1; t < T; ++t) m = max(m, acc[t][r]);
+    any |= m >= H4[r];
+  }
+  return ballot(any) != 0;
+}
 
 template <int KC, int T>
 __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
@@ -657,6 +717,9 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
       xid[t] = a.ids[row];
     }
     const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
+    v4i seed[T];
+    uint32_t vmask, oddm;
+    row_seeds<T>(xn, rows, seed, vmask, oddm);
     const uint32_t *ql = a.qlist + d.x;
     // query of lane c16 in fragment f (clamped into the item)
     auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
@@ -682,30 +745,23 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
       qid1 = qid2;
       if (f + 1 < nf) load_q(qid1, An, qnn, lbn);
       if (f + 2 < nf) qid2 = qid_of(f + 2);
-      // lane c16's query: h = qn - L, or INT32_MAX past the item
-      const int32_t h = 16 * f + c16 < nq ? qn - lb : INT32_MAX;
+      // lane c16's query: H = floor((qn - L) / 2), or 2^30 (above every
+      // accumulator) past the item
+      const int32_t H = 16 * f + c16 < nq ? (qn - lb) >> 1 : (1 << 30);
       v4i acc[T];
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = v4i{0, 0, 0, 0};
+      for (int t = 0; t < T; ++t) acc[t] = mfma(A[0], b[t * KC], seed[t]);
 #pragma unroll
-      for (int kc = 0; kc < KC; ++kc)
+      for (int kc = 1; kc < KC; ++kc)
 #pragma unroll
         for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
-      // h of the lane's 4 queries 4 g16 + r
-      int32_t h4[4];
+      // H of the lane's 4 query slots 4 g16 + r
+      int32_t H4[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) h4[r] = __shfl(h, 4 * g16 + r);
-      bool any = false;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int32_t m = acc[0][r] - (xn[0] >> 1);
-#pragma unroll
-        for (int t = 1; t < T; ++t) m = max(m, acc[t][r] - (xn[t] >> 1));
-        any |= m >= (h4[r] >> 1);
-      }
-      if (!ballot(any)) continue;
-      n_kept += keep_fragment<T>(
-          ks, a, acc, xn, xid, rows, h4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
+      for (int r = 0; r < 4; ++r) H4[r] = __shfl(H, 4 * g16 + r);
+      if (!fragment_passes<T>(acc, H4)) continue;
+      n_kept += keep_exact<T>(
+          ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
           [&](int j) { return __shfl(qn, j); });
     }
   }
@@ -761,6 +817,9 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
       xn[t] = a.xnorm[row];
       xid[t] = a.ids[row];
     }
+    v4i seed[T];
+    uint32_t vmask, oddm;
+    row_seeds<T>(xn, rows, seed, vmask, oddm);
     // producer: this wave's pieces of fragment f (query id q of lane c16),
     // and for wave 0 the fragment's norms and bounds
     auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
@@ -821,25 +880,17 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
         const v4i hv = *reinterpret_cast<const v4i *>(&sH[st][4 * g16]);
         v4i acc[T];
 #pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = v4i{0, 0, 0, 0};
+        for (int t = 0; t < T; ++t) acc[t] = mfma(A[0], b[t * KC], seed[t]);
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
+        for (int kc = 1; kc < KC; ++kc)
 #pragma unroll
           for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
-        int32_t h4[4];
+        int32_t H4[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) h4[r] = 16 * g + 4 * g16 + r < nq ? hv[r] : INT32_MAX;
-        bool any = false;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int32_t m = acc[0][r] - (xn[0] >> 1);
-#pragma unroll
-          for (int t = 1; t < T; ++t) m = max(m, acc[t][r] - (xn[t] >> 1));
-          any |= m >= (h4[r] >> 1);
-        }
-        if (ballot(any))
-          n_kept += keep_fragment<T>(ks, a, acc, xn, xid, rows, h4, [&](int j) { return sQ[st][j]; },
-                                     [&](int j) { return sN[st][j]; });
+        for (int r = 0; r < 4; ++r) H4[r] = 16 * g + 4 * g16 + r < nq ? hv[r] >> 1 : (1 << 30);
+        if (fragment_passes<T>(acc, H4))
+          n_kept += keep_exact<T>(ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return sQ[st][j]; },
+                                  [&](int j) { return sN[st][j]; });
       }
       // fragment g + 1 into the other stage, then the ring moves on
       if (g + 1 < nf) store_frag(ring[0], st ^ 1);
@@ -1196,8 +1247,8 @@ bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, 
 uint32_t rank_wave_rows(uint32_t P, uint32_t variant) {
   const uint32_t kc = P / 64;
   if (kc == 0 || kc > 16 || P % 64 || variant > 1) return 0;
-  static const uint8_t tv[17][2] = {{0, 0}, {8, 4}, {8, 4}, {8, 4}, {6, 4}, {4, 2}, {4, 2}, {4, 2}, {4, 2},
-                                     {2, 1}, {2, 1}, {2, 1}, {2, 1}, {2, 1}, {1, 1}, {1, 1}, {1, 1}};
+  static const uint8_t tv[17][2] = {{0, 0}, {8, 4}, {8, 4}, {8, 4}, {6, 4}, {4, 2}, {4, 2}, {4, 2}, {2, 1},
+                                     {2, 1}, {2, 1}, {2, 1}, {2, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}};
   return 16u * tv[kc][variant];
 }
 
@@ -1229,9 +1280,9 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
       wave_launch<kc, t>(a, grid, s); \
     return true;
     W_CASE(1, 8) W_CASE(1, 4) W_CASE(2, 8) W_CASE(2, 4) W_CASE(3, 8) W_CASE(3, 4) W_CASE(4, 6) W_CASE(4, 4)
-    W_CASE(5, 4) W_CASE(5, 2) W_CASE(6, 4) W_CASE(6, 2) W_CASE(7, 4) W_CASE(7, 2) W_CASE(8, 4) W_CASE(8, 2)
+    W_CASE(5, 4) W_CASE(5, 2) W_CASE(6, 4) W_CASE(6, 2) W_CASE(7, 4) W_CASE(7, 2) W_CASE(8, 2) W_CASE(8, 1)
     W_CASE(9, 2) W_CASE(9, 1) W_CASE(10, 2) W_CASE(10, 1) W_CASE(11, 2) W_CASE(11, 1) W_CASE(12, 2) W_CASE(12, 1)
-    W_CASE(13, 2) W_CASE(13, 1) W_CASE(14, 1) W_CASE(15, 1) W_CASE(16, 1)
+    W_CASE(13, 1) W_CASE(14, 1) W_CASE(15, 1) W_CASE(16, 1)
 #undef W_CASE
     default:
       return false;

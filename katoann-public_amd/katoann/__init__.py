@@ -370,7 +370,7 @@ class Index:
         _check(lib().katoann_set_rank_rows(self._h, rows))
 
     def set_band_factor(self, log2_factor: int):
-        """Probe-rank bands of the ranking pass grow by 2**log2_factor (1..4)."""
+        """Probe-rank bands of the ranking pass grow by 2**log2_factor (1..12)."""
         _check(lib().katoann_set_band_factor(self._h, log2_factor))
 
     def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 512):

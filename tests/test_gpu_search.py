@@ -215,7 +215,7 @@ def test_rank_kernel_choice_is_invisible(mixture_index):
     ix = load(prefix)
     ref = None
     for (kernel, variant, qc) in RANK_KERNELS:
-        for shift in (1, 2, 4):
+        for shift in (1, 2, 4, 7):
             ix.set_rank_kernel(kernel, variant, qc)
             ix.set_band_factor(shift)
             print("choice", kernel, variant, qc, shift, flush=True)
