@@ -178,7 +178,10 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
  * streamed), 1 = wave items (a wave keeps up to 16 T rows of a list in
  * registers and streams up to queries_per_item of its queries), 2 = block items
  * of 4 x 16 T rows whose 4 waves share the queries through an LDS ring;
- * variant 0 or 1 picks the larger or smaller T. Default: 1, variant 0, 512. */
+ * variant 0 or 1 picks the larger or smaller T; variant 2 (kernel 1 only) a
+ * smaller T with query codes loaded two fragments ahead instead of one;
+ * variant 3 (kernel 1 only) a larger T at one wave per SIMD.
+ * Default: 1, variant 0, 512. */
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item);
 /* Graph replay (no reference counterpart): each search context captures its
  * kernel sequence (normalisation through top-k) into a hipGraph once and

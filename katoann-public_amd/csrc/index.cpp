@@ -87,7 +87,7 @@ struct SearchCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[KATOANN_NSTAGES] = {};  // stage boundaries (the pool updates are timed as the rest of the ranking interval)
   hipEvent_t bev[2 * kMaxBands] = {};   // around each band's ranking kernel
-  DevBuf q, y, qhi, qlo, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
+  DevBuf q, y, qhi, qlo, qpre, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
       totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items, mx16, mh1,
       mh2, mz;
   std::vector<uint32_t> host_totals;
@@ -432,10 +432,10 @@ struct SearchPlan {
   uint32_t B, d, dc, np, K, S, nlist, P, shift, qc, nslots, nbands, nstate, rt, item_cap, variant;
   bool prefix, wave, ring;
   uint64_t max_rows, allowance;
-  size_t q, qcodes, qnorm, q4, dist, probes, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
+  size_t q, qcodes, qpre, qnorm, q4, dist, probes, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
       codes, npq, mx16, mh, pool;
   size_t total() const {
-    return 2 * q + 2 * qcodes + qnorm + 2 * q4 + dist + 3 * probes + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
+    return 2 * q + 2 * qcodes + qpre + qnorm + 2 * q4 + dist + 3 * probes + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
            q4 /* pool_sorted */ + thr + q4 /* lbound */ + state + items + 2 * out + surv + (surv ? q4 : 0) + codes +
            npq + mx16 + 2 * mh + (mx16 ? q4 : 0) + pool;
   }
@@ -514,6 +514,10 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   const size_t B = z.B;
   z.q = B * z.d * 4;
   z.qcodes = B * z.dc;
+  // the queries' prefix bytes, compact (rows P bytes apart): the ranking
+  // kernels gather them, and rows dc bytes apart would fall on a few L2
+  // channels only
+  z.qpre = z.prefix ? B * z.P : 0;
   z.qnorm = B * 8;
   z.q4 = B * 4;
   z.dist = B * z.nlist * 4;
@@ -559,6 +563,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.y.ensure(z.q);
   c.qhi.ensure(z.qcodes);
   c.qlo.ensure(z.qcodes);
+  c.qpre.ensure(z.qpre);
   c.qnorm.ensure(z.qnorm);
   c.qhn.ensure(z.q4);
   c.qh0.ensure(z.q4);
@@ -605,7 +610,9 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   auto mark = [&](int i) {
     if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
   };
-  const uint32_t grid = static_cast<uint32_t>(device_cus()) * 2;  // ranking blocks resident per CU
+  // ranking blocks: as many as are resident at once
+  const uint32_t grid = static_cast<uint32_t>(device_cus()) *
+                        (z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant) : 2u);
   uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
   float *ddst = dev_io ? out_dists : c.dists.as<float>();
   const float *yq = c.q.as<float>();
@@ -626,7 +633,8 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     }
     katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
                                  c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(),
-                                 c.qh0.as<int32_t>(), nullptr, (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
+                                 c.qh0.as<int32_t>(), prefix ? c.qpre.as<int8_t>() : nullptr,
+                                 (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
     mark(1);
     katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
                                ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist,
@@ -666,16 +674,18 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
     const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
+    const int8_t *qrank = prefix ? c.qpre.as<int8_t>() : c.qhi.as<int8_t>();
+    const uint32_t qstride = prefix ? P : dc;
     for (uint32_t band = 0; band < nbands; ++band) {
       if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
       const bool launched =
           z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
-                                                 c.qhi.as<int8_t>(), dc, qn, c.lbound.as<int32_t>(),
+                                                 qrank, qstride, qn, c.lbound.as<int32_t>(),
                                                  c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
                                                  c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
                                                  c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s)
                  : katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
-                                                 c.qhi.as<int8_t>(), dc, qn, c.lbound.as<int32_t>(),
+                                                 qrank, qstride, qn, c.lbound.as<int32_t>(),
                                                  c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
                                                  c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
                                                  c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
@@ -708,7 +718,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                    ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
                                    x && x->survivors, x && x->n_survivors, x && x->codes};
-      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
+      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qpre, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
                               &c.npq, &c.gcnt, &c.qoff, &c.pscratch, &c.item_off, &c.qlist, &c.totals, &c.pool_cnt,
                               &c.pool_sorted, &c.thr, &c.lbound, &c.state, &c.pool, &c.surv, &c.nsurv, &c.codes,
                               &c.items, &c.mx16, &c.mh1, &c.mh2, &c.mz})
@@ -948,7 +958,7 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
     trial(TileConfig{1u, 0u, 1024u, shift});
   }
   const uint32_t shift = pick.shift;
-  for (uint32_t variant : {0u, 1u})
+  for (uint32_t variant : {0u, 1u, 2u, 3u})
     for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift});
   for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift});
   std::lock_guard<std::mutex> g(mix.tune_mu);
@@ -1099,7 +1109,8 @@ int katoann_set_graphs(katoann_index *h, int32_t on) {
 
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item) {
   KATOANN_TRY
-  if (kernel > 2 || variant > 1) throw std::invalid_argument("rank kernel is 0, 1 or 2, variant 0 or 1");
+  if (kernel > 2 || variant > 3 || (kernel != 1 && variant > 1))
+    throw std::invalid_argument("rank kernel is 0, 1 or 2; variant 0, 1 or (kernel 1) 2 or 3");
   if (queries_per_item == 0 || queries_per_item > 65536)
     throw std::invalid_argument("queries per ranking item must be in 1..65536");
   std::lock_guard<std::mutex> g(h->mu);

@@ -53,6 +53,9 @@ bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, 
 // Wave-item ranking kernels (wave-independent, or a block of 4 waves sharing a
 // query ring when `ring`): rows per wave for ranking width P and register
 // variant 0 (large) or 1 (half); 0 if unsupported.
+// Blocks of the wave kernel resident per CU (1 for the tiles that take a whole
+// SIMD's registers per wave).
+uint32_t rank_wave_blocks_per_cu(uint32_t P, uint32_t variant);
 uint32_t rank_wave_rows(uint32_t P, uint32_t variant);
 bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
                       uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,

@@ -54,7 +54,7 @@ struct BandArgs {
   const uint32_t *ids;       // N
   uint32_t nlist;
   uint32_t band;
-  const int8_t *qhi;         // B x dc
+  const int8_t *qhi;         // B rows dc bytes apart, the ranked bytes first
   uint32_t dc;
   const int32_t *qnorm;      // B
   const int32_t *lbound;     // B: largest integer l2 whose key can reach the query's threshold
@@ -490,7 +490,7 @@ struct WaveArgs {
   const int8_t *plane;       // N x P, row-major, shifted high bytes
   const int32_t *xnorm;      // N
   const uint32_t *ids;       // N
-  const int8_t *qhi;         // B x dc
+  const int8_t *qhi;         // B rows dc bytes apart, the ranked bytes first
   uint32_t dc;
   const int32_t *qnorm;      // B
   const int32_t *lbound;     // B
@@ -524,10 +524,15 @@ struct KeepStage {
   uint32_t base, q;    // lane 16 g + r (r < 4): pool base and query of slot 4 g + r
 };
 
+// (A loop with a fixed trip count and at most one overflow atomic, so the
+// compiler can count the wave's outstanding memory operations across it.)
 __device__ __forceinline__ void keep_drain(KeepStage &ks, const WaveArgs &a) {
   const int lane = lane_id();
-  for (uint32_t e0 = 0; e0 < ks.pending; e0 += 64) {
-    const uint32_t e = e0 + lane;
+  bool ovf = false;
+#pragma unroll
+  for (int c = 0; c < KCAP / 64; ++c) {
+    if (static_cast<uint32_t>(64 * c) >= ks.pending) break;
+    const uint32_t e = 64 * c + lane;
     const bool v = e < ks.pending;
     const uint32_t m = v ? ks.meta[ks.par * KCAP + e] : 0u;
     const uint64_t key = v ? ks.key[ks.par * KCAP + e] : 0ull;
@@ -536,9 +541,10 @@ __device__ __forceinline__ void keep_drain(KeepStage &ks, const WaveArgs &a) {
     const uint32_t q = __shfl(ks.q, src);
     if (v) {
       if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
-      else atomicOr(&a.state[0], 1u);
+      else ovf = true;
     }
   }
+  if (ovf) atomicOr(&a.state[0], 1u);
   ks.pending = 0;
 }
 
@@ -584,6 +590,11 @@ __device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a,
                                                uint32_t vmask, uint32_t oddm, const uint32_t (&xid)[T],
                                                const int32_t (&H4)[4], QS qslot, QN qnorm_slot) {
   const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  // norms of the lane's query slots, fetched first so the shuffles' latency
+  // overlaps the tests below
+  int32_t qnr[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) qnr[r] = qnorm_slot(4 * g16 + r);
   uint32_t bits[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -621,12 +632,12 @@ __device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a,
     if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
   }
   const int reg = ks.par ^ 1;
+  bool ovf = false;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int j = 4 * g16 + r;
-    const int32_t qnr = qnorm_slot(j);
-    const uint32_t q = qslot(j);
-    const uint32_t boff = __shfl(base, 16 * g16 + r);
+    const uint32_t q = direct ? qslot(j) : 0u;
+    const uint32_t boff = direct ? __shfl(base, 16 * g16 + r) : 0u;
     uint32_t idx = (ex >> (8 * r)) & 0xffu;  // the lane's first key within the slot's run
     int32_t av[T];
 #pragma unroll
@@ -635,12 +646,12 @@ __device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a,
     while (m) {
       const int t = __builtin_ctz(m);
       m &= m - 1u;
-      const int32_t l2 = qnr - 2 * pick(av, t) - static_cast<int32_t>((oddm >> t) & 1u);
+      const int32_t l2 = qnr[r] - 2 * pick(av, t) - static_cast<int32_t>((oddm >> t) & 1u);
       const uint64_t key = make_key(fscale(l2, a.step_sq), pick(xid, t));
       if (direct) {
         const uint32_t pos = boff + idx;
         if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
-        else atomicOr(&a.state[0], 1u);
+        else ovf = true;
       } else {
         ks.key[reg * KCAP + so[r] + idx] = key;
         ks.meta[reg * KCAP + so[r] + idx] = static_cast<uint32_t>(j) | (idx << 4);
@@ -648,7 +659,10 @@ __device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a,
       ++idx;
     }
   }
-  if (direct) return total;
+  if (direct) {
+    if (ovf) atomicOr(&a.state[0], 1u);
+    return total;
+  }
   if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
   keep_drain(ks, a);
   ks.par = reg;
@@ -689,8 +703,17 @@ __device__ __forceinline__ bool fragment_passes(const v4i (&acc)[T], const int32
   return ballot(any) != 0;
 }
 
-template <int KC, int T>
-__global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
+// One query fragment of the wave kernel: the lane's A operands, its query id,
+// norm and bound.
+template <int KC>
+struct QFrag {
+  v4i A[KC];
+  int32_t qn, lb;
+  uint32_t q;
+};
+
+template <int KC, int T, int NB>
+__global__ __launch_bounds__(256, (T * KC > 32 ? 1 : 2)) void k_rank_wave(WaveArgs a) {
   constexpr uint32_t P = 64 * KC;
   __shared__ uint64_t s_key[4][2 * KCAP];
   __shared__ uint32_t s_meta[4][2 * KCAP];
@@ -703,6 +726,21 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
   const uint32_t nw = gridDim.x * 4;
   for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw) {
     const uint4 d = a.items[it];
+    const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
+    const uint32_t *ql = a.qlist + d.x;
+    // query of lane c16 in fragment f (clamped into the item)
+    auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
+    // Query fragments stream through a ring of NB register buffers, codes
+    // loaded NB - 1 fragments ahead and ids NB ahead; the loop is unrolled by
+    // NB so no buffer is ever copied (a copy would wait for the loads just
+    // issued). Ids are loaded before the loads that follow them (the rows,
+    // a step's codes), so waiting for an id waits for nothing issued after
+    // it; fragments past the item load its last query again (unconditional
+    // loads leave no register copies). Norms and bounds stay raw until the
+    // fragment is ranked.
+    uint32_t qfirst[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) qfirst[u] = qid_of(u);
     // the item's rows: B operands, norms and ids
     v4i b[T * KC];
     int32_t xn[T];
@@ -716,53 +754,53 @@ __global__ __launch_bounds__(256, 2) void k_rank_wave(WaveArgs a) {
       xn[t] = a.xnorm[row];
       xid[t] = a.ids[row];
     }
-    const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
+    QFrag<KC> Q[NB];
+    auto load_q = [&](QFrag<KC> &B, uint32_t q) {
+      B.q = q;
+      const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) B.A[kc] = load16(src + 64 * kc);
+      B.qn = a.qnorm[q];
+      B.lb = a.lbound[q];
+    };
+    uint32_t qnext = qfirst[NB - 1];
+#pragma unroll
+    for (int u = 0; u + 1 < NB; ++u) load_q(Q[u], qfirst[u]);
     v4i seed[T];
     uint32_t vmask, oddm;
     row_seeds<T>(xn, rows, seed, vmask, oddm);
-    const uint32_t *ql = a.qlist + d.x;
-    // query of lane c16 in fragment f (clamped into the item)
-    auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
-    uint32_t qid1 = qid_of(0), qid2 = nf > 1 ? qid_of(1) : qid1;
-    v4i An[KC];
-    int32_t qnn, lbn;
-    // norms and bounds stay raw until the fragment is ranked: using them here
-    // would wait for the loads just issued
-    auto load_q = [&](uint32_t q, v4i(&A)[KC], int32_t &qn, int32_t &lb) {
-      const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) A[kc] = load16(src + 64 * kc);
-      qn = a.qnorm[q];
-      lb = a.lbound[q];
-    };
-    load_q(qid1, An, qnn, lbn);
-    for (uint32_t f = 0; f < nf; ++f) {
-      v4i A[KC];
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) A[kc] = An[kc];
-      const int32_t qn = qnn, lb = lbn;
-      const uint32_t qid = qid1;
-      qid1 = qid2;
-      if (f + 1 < nf) load_q(qid1, An, qnn, lbn);
-      if (f + 2 < nf) qid2 = qid_of(f + 2);
+    // fragment f from buffer C; fragment f + NB - 1's codes into buffer N
+    auto step = [&](uint32_t f, const QFrag<KC> &C, QFrag<KC> &N) {
+      const uint32_t qa = qnext;
+      qnext = qid_of(f + NB);
+      load_q(N, qa);
       // lane c16's query: H = floor((qn - L) / 2), or 2^30 (above every
       // accumulator) past the item
-      const int32_t H = 16 * f + c16 < nq ? (qn - lb) >> 1 : (1 << 30);
+      const int32_t H = 16 * f + c16 < nq ? (C.qn - C.lb) >> 1 : (1 << 30);
       v4i acc[T];
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = mfma(A[0], b[t * KC], seed[t]);
+      for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[0], b[t * KC], seed[t]);
 #pragma unroll
       for (int kc = 1; kc < KC; ++kc)
 #pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = mfma(A[kc], b[t * KC + kc], acc[t]);
+        for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[kc], b[t * KC + kc], acc[t]);
       // H of the lane's 4 query slots 4 g16 + r
       int32_t H4[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) H4[r] = __shfl(H, 4 * g16 + r);
-      if (!fragment_passes<T>(acc, H4)) continue;
+      if (!fragment_passes<T>(acc, H4)) return;
+      const uint32_t qid = C.q;
+      const int32_t qn = C.qn;
       n_kept += keep_exact<T>(
           ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
           [&](int j) { return __shfl(qn, j); });
+    };
+    for (uint32_t f0 = 0; f0 < nf; f0 += NB) {
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        if (f0 + u >= nf) break;
+        step(f0 + u, Q[u], Q[(u + NB - 1) % NB]);
+      }
     }
   }
   keep_drain(ks, a);
@@ -1244,17 +1282,38 @@ bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, 
 
 // Rows per wave of the wave-independent ranking kernel: 16 T, with T one of
 // two register budgets per ranking width (B operands of 32 or 16 x 16 bytes).
+// Wave-kernel tiles per prefix chunk count KC = P / 64 and variant: rows per
+// wave (16 T) and query buffers in the ring (NB). Variants 0 and 1 take the
+// larger and smaller T with a two-buffer ring; variant 2 a three-buffer ring
+// (codes two fragments ahead) for KC <= 8, and variant 0's tile above;
+// variant 3 for KC = 4 twice variant 0's rows at one wave per SIMD (the
+// kernel's launch bounds), and variant 0's tile otherwise. The ring kernel
+// takes T of variants 0 and 1.
+struct WaveTile {
+  uint8_t t, nb;
+};
+static const WaveTile kWaveTiles[17][4] = {
+    {{0, 0}, {0, 0}, {0, 0}, {0, 0}}, {{8, 2}, {4, 2}, {8, 3}, {8, 2}}, {{8, 2}, {4, 2}, {8, 3}, {8, 2}},
+    {{6, 2}, {4, 2}, {6, 3}, {6, 2}}, {{6, 2}, {4, 2}, {5, 3}, {12, 4}}, {{4, 2}, {2, 2}, {3, 3}, {4, 2}},
+    {{4, 2}, {2, 2}, {3, 3}, {4, 2}}, {{3, 2}, {2, 2}, {2, 3}, {3, 2}}, {{2, 2}, {1, 2}, {2, 3}, {2, 2}},
+    {{2, 2}, {1, 2}, {2, 2}, {2, 2}}, {{2, 2}, {1, 2}, {2, 2}, {2, 2}}, {{1, 2}, {1, 2}, {1, 2}, {1, 2}},
+    {{1, 2}, {1, 2}, {1, 2}, {1, 2}}, {{1, 2}, {1, 2}, {1, 2}, {1, 2}}, {{1, 2}, {1, 2}, {1, 2}, {1, 2}},
+    {{1, 2}, {1, 2}, {1, 2}, {1, 2}}, {{1, 2}, {1, 2}, {1, 2}, {1, 2}}};
+
 uint32_t rank_wave_rows(uint32_t P, uint32_t variant) {
   const uint32_t kc = P / 64;
-  if (kc == 0 || kc > 16 || P % 64 || variant > 1) return 0;
-  static const uint8_t tv[17][2] = {{0, 0}, {8, 4}, {8, 4}, {8, 4}, {6, 4}, {4, 2}, {4, 2}, {4, 2}, {2, 1},
-                                     {2, 1}, {2, 1}, {2, 1}, {2, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}};
-  return 16u * tv[kc][variant];
+  if (kc == 0 || kc > 16 || P % 64 || variant > 3) return 0;
+  return 16u * kWaveTiles[kc][variant].t;
 }
 
-template <int KC, int T>
+uint32_t rank_wave_blocks_per_cu(uint32_t P, uint32_t variant) {
+  const uint32_t t = rank_wave_rows(P, variant) / 16;
+  return t * (P / 64) > 32 ? 1u : 2u;  // the kernel's launch bounds
+}
+
+template <int KC, int T, int NB>
 static void wave_launch(const WaveArgs &a, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_rank_wave<KC, T>), dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_rank_wave<KC, T, NB>), dim3(grid), dim3(256), 0, s, a);
 }
 
 template <int KC, int T>
@@ -1267,22 +1326,36 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
                       const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
                       uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
                       uint32_t grid, hipStream_t s) {
-  const uint32_t T = rank_wave_rows(P, variant) / 16;
-  if (T == 0) return false;
+  if (rank_wave_rows(P, variant) == 0 || (ring && variant > 1)) return false;
+  const WaveTile w = kWaveTiles[P / 64][variant];
   const WaveArgs a{plane, xnorm, ids, qhi, dc, qnorm, lbound, qlist, item_off, items, nlist, stride, pool, pool_cnt,
                    state, band, step_sq};
-  switch (P / 64 * 16 + T) {
-#define W_CASE(kc, t) \
-  case kc * 16 + t:   \
-    if (ring)         \
-      ring_launch<kc, t>(a, grid, s); \
-    else              \
-      wave_launch<kc, t>(a, grid, s); \
+  if (ring) {
+    switch (P / 64 * 16 + w.t) {
+#define R_CASE(kc, t)              \
+  case kc * 16 + t:                \
+    ring_launch<kc, t>(a, grid, s); \
     return true;
-    W_CASE(1, 8) W_CASE(1, 4) W_CASE(2, 8) W_CASE(2, 4) W_CASE(3, 8) W_CASE(3, 4) W_CASE(4, 6) W_CASE(4, 4)
-    W_CASE(5, 4) W_CASE(5, 2) W_CASE(6, 4) W_CASE(6, 2) W_CASE(7, 4) W_CASE(7, 2) W_CASE(8, 2) W_CASE(8, 1)
-    W_CASE(9, 2) W_CASE(9, 1) W_CASE(10, 2) W_CASE(10, 1) W_CASE(11, 2) W_CASE(11, 1) W_CASE(12, 2) W_CASE(12, 1)
-    W_CASE(13, 1) W_CASE(14, 1) W_CASE(15, 1) W_CASE(16, 1)
+      R_CASE(1, 8) R_CASE(1, 4) R_CASE(2, 8) R_CASE(2, 4) R_CASE(3, 6) R_CASE(3, 4) R_CASE(4, 6) R_CASE(4, 4)
+      R_CASE(5, 4) R_CASE(5, 2) R_CASE(6, 4) R_CASE(6, 2) R_CASE(7, 3) R_CASE(7, 2) R_CASE(8, 2) R_CASE(8, 1)
+      R_CASE(9, 2) R_CASE(9, 1) R_CASE(10, 2) R_CASE(10, 1) R_CASE(11, 1) R_CASE(12, 1) R_CASE(13, 1)
+      R_CASE(14, 1) R_CASE(15, 1) R_CASE(16, 1)
+#undef R_CASE
+      default:
+        return false;
+    }
+  }
+  switch ((P / 64 * 16 + w.t) * 4 + w.nb) {
+#define W_CASE(kc, t, nb)                \
+  case (kc * 16 + t) * 4 + nb:           \
+    wave_launch<kc, t, nb>(a, grid, s);  \
+    return true;
+    W_CASE(1, 8, 2) W_CASE(1, 4, 2) W_CASE(1, 8, 3) W_CASE(2, 8, 2) W_CASE(2, 4, 2) W_CASE(2, 8, 3)
+    W_CASE(3, 6, 2) W_CASE(3, 4, 2) W_CASE(3, 6, 3) W_CASE(4, 6, 2) W_CASE(4, 4, 2) W_CASE(4, 5, 3) W_CASE(4, 12, 4)
+    W_CASE(5, 4, 2) W_CASE(5, 2, 2) W_CASE(5, 3, 3) W_CASE(6, 4, 2) W_CASE(6, 2, 2) W_CASE(6, 3, 3)
+    W_CASE(7, 3, 2) W_CASE(7, 2, 2) W_CASE(7, 2, 3) W_CASE(8, 2, 2) W_CASE(8, 1, 2) W_CASE(8, 2, 3)
+    W_CASE(9, 2, 2) W_CASE(9, 1, 2) W_CASE(10, 2, 2) W_CASE(10, 1, 2) W_CASE(11, 1, 2) W_CASE(12, 1, 2)
+    W_CASE(13, 1, 2) W_CASE(14, 1, 2) W_CASE(15, 1, 2) W_CASE(16, 1, 2)
 #undef W_CASE
     default:
       return false;

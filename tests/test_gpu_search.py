@@ -181,8 +181,10 @@ def wide_index(tmp_path_factory):
 
 
 # (kernel, variant, queries per item): block items; wave items with the large and
-# the small row tile; wave items split into many query chunks
-RANK_KERNELS = [(0, 0, 512), (1, 0, 512), (1, 1, 512), (1, 0, 16), (1, 1, 1), (2, 0, 512), (2, 1, 512), (2, 0, 20), (2, 1, 1)]
+# the small row tile and with the deep query ring; wave items split into many
+# query chunks; block items with the query ring
+RANK_KERNELS = [(0, 0, 512), (1, 0, 512), (1, 1, 512), (1, 2, 512), (1, 3, 512), (1, 0, 16), (1, 1, 1), (1, 2, 40),
+                (1, 2, 1), (1, 3, 24), (2, 0, 512), (2, 1, 512), (2, 0, 20), (2, 1, 1)]
 
 
 _ORACLES = {}
