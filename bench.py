@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--rank-rows", type=int, default=0)
     ap.add_argument("--band-shift", type=int, default=0, help="probe-rank bands grow by 2**shift (default: library's)")
     ap.add_argument("--rank-kernel", default="", help="kernel,variant,queries_per_item (default: library's)")
+    ap.add_argument("--seed-rows", type=int, default=-1, help="rows of the seed band (0 = none; default: library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1024)
     ap.add_argument("--no-learned", action="store_true")
@@ -127,6 +128,8 @@ def main():
         ix.set_rank_rows(args.rank_rows)
     if args.band_shift:
         ix.set_band_factor(args.band_shift)
+    if args.seed_rows >= 0:
+        ix.set_seed_rows(args.seed_rows)
     if args.rank_kernel:
         ix.set_rank_kernel(*[int(x) for x in args.rank_kernel.split(",")])
     else:

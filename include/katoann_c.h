@@ -174,6 +174,18 @@ int katoann_set_rank_rows(katoann_index *h, uint32_t rows);
  * (F^(b+1) - 1) / (F - 1)) with F = 2^log2_factor (1..12); the queries' thresholds
  * tighten between bands. Default 2 (F = 4). */
 int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
+/* Seed band (no reference counterpart): with rows = R0 > 0 the ranking pass
+ * first ranks each query's nearest lists up to R0 rows in all (at most the
+ * first R0 rows of each; R0 is raised to max(survivors, k)), then the rest of
+ * those lists, then probe-rank bands of factor F from the next rank. Every
+ * query thus has a threshold before its wider bands. 0 (default) = bands from
+ * rank 0. Results do not depend on it. */
+int katoann_set_seed_rows(katoann_index *h, uint32_t rows);
+/* Survivor pool room per query beyond S for one band's kept keys (no
+ * reference counterpart): 0 (default) sizes it per search. A band that keeps
+ * more runs the batch again with 4x the room; later searches of the shape
+ * start from the room that sufficed. Results do not depend on it. */
+int katoann_set_pool_allowance(katoann_index *h, uint32_t slots);
 /* Ranking kernel: 0 = block items (a list's queries staged into LDS, rows
  * streamed), 1 = wave items (a wave keeps up to 16 T rows of a list in
  * registers and streams up to queries_per_item of its queries), 2 = block items
@@ -191,12 +203,12 @@ int katoann_set_graphs(katoann_index *h, int32_t on);
 /* Tile autotuning (no reference counterpart): with it on, the first search of
  * each shape (ranking width, nlist, batch rounded up to a power of two,
  * nprobe, max(survivors, k)) times the candidate ranking tiles (kernel, row
- * tile, queries per item, band factor) on its own inputs and keeps the fastest
+ * tile, queries per item, band factor, seed rows) on its own inputs and keeps the fastest
  * for that shape; results are the same whatever the tile. Default off. */
 int katoann_set_autotune(katoann_index *h, int32_t on);
 /* The tile a search of this shape runs with: kernel, variant, queries per
- * item, log2 band factor. */
-int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out4);
+ * item, log2 band factor, seed rows (out: 5 entries). */
+int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out5);
 
 /* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
  * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of

@@ -114,6 +114,7 @@ struct SearchCtx {
 // items with a query ring), row-tile variant, queries per item, log2 band factor.
 struct TileConfig {
   uint32_t kernel, variant, qc, shift;
+  uint32_t seed;  // rows of the seed band (0: geometric bands from rank 0)
 };
 using TileKey = std::array<uint32_t, 6>;  // P, nlist, batch rounded up to 2^k, nprobe, S, prefix
 
@@ -128,11 +129,15 @@ struct katoann_index {
   uint64_t nrows = 0;
   uint32_t rank_rows = 1024;
   uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
+  uint32_t seed_rows = 0;   // rows of the seed band (0: none)
+  uint32_t pool_allowance = 0;  // pool slots per query beyond S (0: sized per search)
   uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 512;
   std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
   std::atomic<bool> autotune{false};
   std::mutex tune_mu;
   std::map<TileKey, TileConfig> tuned;  // per shape, the fastest tile measured
+  // per shape and tile, the pool allowance the last overflow rerun needed
+  std::map<std::array<uint64_t, 8>, uint64_t> allowance_seen;
   uint64_t gen = 0;                // bumped by every load: captured graphs name the old buffers
   std::vector<uint64_t> tile_prefix;  // sum of the p largest tile counts, p = 0..nlist
   uint32_t tile_prefix_rows = 0;
@@ -152,6 +157,7 @@ struct katoann_index {
     {
       std::lock_guard<std::mutex> g(tune_mu);
       tuned.clear();
+      allowance_seen.clear();
     }
     dim = nlist = npts = dc = head_dims = 0;
     normalized = rotated = false;
@@ -429,13 +435,13 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // Validated shape of one search and the bytes of every per-search buffer, so
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
-  uint32_t B, d, dc, np, K, S, nlist, P, shift, qc, nslots, nbands, nstate, rt, item_cap, variant;
+  uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
   bool prefix, wave, ring;
   uint64_t max_rows, allowance;
-  size_t q, qcodes, qpre, qnorm, q4, dist, probes, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
+  size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
       codes, npq, mx16, mh, pool;
   size_t total() const {
-    return 2 * q + 2 * qcodes + qpre + qnorm + 2 * q4 + dist + 3 * probes + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
+    return 2 * q + 2 * qcodes + qpre + qnorm + 2 * q4 + dist + 2 * probes + qlist + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
            q4 /* pool_sorted */ + thr + q4 /* lbound */ + state + items + 2 * out + surv + (surv ? q4 : 0) + codes +
            npq + mx16 + 2 * mh + (mx16 ? q4 : 0) + pool;
   }
@@ -459,7 +465,7 @@ TileConfig tile_for(const katoann_index &ix, uint32_t nq, const katoann_search_p
     auto it = mix.tuned.find(tile_key(ix, nq, p));
     if (it != mix.tuned.end()) return it->second;
   }
-  return TileConfig{ix.rank_kernel, ix.wave_variant, ix.wave_qc, ix.band_shift};
+  return TileConfig{ix.rank_kernel, ix.wave_variant, ix.wave_qc, ix.band_shift, ix.seed_rows};
 }
 
 SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_search_params &p,
@@ -486,14 +492,16 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.B = nq, z.d = ix.dim, z.dc = ix.dc, z.np = p.nprobe, z.K = p.k, z.nlist = ix.nlist;
   const TileConfig tc = tile_for(ix, nq, p, force);
   z.shift = tc.shift;
+  // a seed band below S rows could leave a query without a threshold
+  z.seed = tc.seed ? std::max(tc.seed, std::max(p.survivors, p.k)) : 0u;
   z.wave = tc.kernel >= 1;
   z.ring = tc.kernel == 2;
   z.variant = tc.variant;
   z.qc = z.wave ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
   z.rt = z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1) : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
-  z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift);
-  z.nbands = katoann_hip::plan_bands(z.np, z.shift);
+  z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift, z.seed);
+  z.nbands = katoann_hip::plan_bands(z.np, z.shift, z.seed);
   if (z.nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
   z.nstate = 3 + z.nbands;
   auto &mix = const_cast<katoann_index &>(ix);
@@ -502,15 +510,21 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   // groups over all bands hold at most B queries
   uint64_t tiles_total = 0;
   for (uint32_t cc = 0; cc < z.nlist; ++cc) tiles_total += (ix.sizes[cc] + z.rt - 1) / z.rt;
-  const uint64_t item_cap64 = tiles_total * ((z.B + z.qc - 1) / z.qc + z.nbands);
+  // (a seed band splits each list's rows in two: one more tile per list)
+  const uint64_t item_cap64 = tiles_total * ((z.B + z.qc - 1) / z.qc + z.nbands) +
+                              (z.seed ? static_cast<uint64_t>(z.nlist) * ((z.B + z.qc - 1) / z.qc + 1) : 0);
   if (item_cap64 > 0x7fffffffull) throw std::invalid_argument("batch too large for the ranking plan");
   z.item_cap = static_cast<uint32_t>(std::max<uint64_t>(item_cap64, 1));
   // the first bands run without a threshold (or with one from few candidates):
   // room for every row of a query's 1 + F nearest lists
-  z.allowance = std::min(
-      z.max_rows,
-      std::max<uint64_t>({kPoolAllowance, 2 * mix.max_rows_per_query(1),
-                          std::min(kPoolAllowanceCap, mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift))))}));
+  // (with a seed band: at most 2 R0 keys before the first threshold)
+  z.allowance =
+      z.seed ? std::min<uint64_t>(z.max_rows, std::max<uint64_t>(kPoolAllowance, 2ull * z.seed))
+             : std::min(z.max_rows,
+                        std::max<uint64_t>({kPoolAllowance, 2 * mix.max_rows_per_query(1),
+                                            std::min(kPoolAllowanceCap,
+                                                     mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift))))}));
+  if (ix.pool_allowance) z.allowance = std::min<uint64_t>(z.max_rows, ix.pool_allowance);
   const size_t B = z.B;
   z.q = B * z.d * 4;
   z.qcodes = B * z.dc;
@@ -523,7 +537,9 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.dist = B * z.nlist * 4;
   z.probes = B * z.np * 4;
   z.gcnt = z.nslots * 4ull;
-  z.pscratch = katoann_hip::plan_scratch_words(z.B, z.nlist, z.np, z.shift) * 4;
+  z.pscratch = katoann_hip::plan_scratch_words(z.B, z.nlist, z.np, z.shift, z.seed) * 4;
+  // query lists: every probed pair once, seed pairs with rows past R0 twice
+  z.qlist = z.probes * (z.seed ? 2 : 1);
   z.item_off = (z.nslots + 1ull) * 4;
   z.pool_cnt = B * 4 * katoann_hip::kCntStride;
   z.thr = B * 8;
@@ -575,7 +591,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.qoff.ensure(z.gcnt);
   c.pscratch.ensure(z.pscratch);
   c.item_off.ensure(z.item_off);
-  c.qlist.ensure(z.probes);
+  c.qlist.ensure(z.qlist);
   c.totals.ensure(16);
   c.pool_cnt.ensure(z.pool_cnt);
   c.pool_sorted.ensure(z.q4);
@@ -667,9 +683,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
                              c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
                              c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(), c.thr.as<uint64_t>(),
-                             c.lbound.as<int32_t>(), c.state.as<uint32_t>(), nstate, shift, s);
+                             c.lbound.as<int32_t>(), c.state.as<uint32_t>(), nstate, shift, z.seed, s);
     katoann_hip::launch_items(c.item_off.as<uint32_t>(), nslots, nlist, ix.list_off_d.as<uint64_t>(),
-                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), qc, rt, item_cap, c.items.as<uint4>(), s);
+                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), qc, rt, item_cap, z.seed, c.items.as<uint4>(),
+                              s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
@@ -713,7 +730,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                             dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     if (graphs) {
       // everything a launch reads: shapes, scalars, every buffer, the stream
-      std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, qc,
+      std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
                                    rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
@@ -760,14 +777,27 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     HIPCHECK(hipStreamSynchronize(s));
     return st[0] == 0;
   };
-  // on a pool overflow the batch runs again with 4x the room, up to every candidate
+  // on a pool overflow the batch runs again with 4x the room, up to every
+  // candidate; later searches of the shape start from the room that sufficed
+  const std::array<uint64_t, 8> akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
+                                        uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9)};
   uint64_t allowance = z.allowance;
+  {
+    std::lock_guard<std::mutex> g(mix.tune_mu);
+    auto it = mix.allowance_seen.find(akey);
+    if (it != mix.allowance_seen.end()) allowance = std::max(allowance, it->second);
+  }
+  const uint64_t first = allowance;
   while (!run(allowance)) {
     if (allowance >= max_rows) {
       ix.release(std::move(ctx));
       throw std::runtime_error("survivor pool overflow in the ranking pass");
     }
     allowance = std::min<uint64_t>(max_rows, allowance * 4);
+  }
+  if (allowance != first) {
+    std::lock_guard<std::mutex> g(mix.tune_mu);
+    mix.allowance_seen[akey] = allowance;
   }
 
   const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -923,8 +953,9 @@ int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t
 
 namespace {
 
-// First search of a shape with autotuning on: every candidate tile runs twice
-// (the first run captures its graph), the second is timed; the fastest is kept
+// First search of a shape with autotuning on: every candidate tile runs three
+// times (the first captures its graph), the better of the other two counts;
+// the fastest is kept
 // for the shape. Every tile gives the same results, so the last run's outputs
 // stand.
 void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
@@ -937,30 +968,37 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
     if (mix.tuned.count(key)) return;
   }
   double best = 1e30;
-  TileConfig pick{1u, 0u, 1024u, 3u};
+  TileConfig pick{1u, 0u, 1024u, 3u, 0u};
   auto trial = [&](const TileConfig &c) {
     if (katoann_hip::rank_wave_rows(key[0], c.variant) == 0) return;
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
-    const auto t0 = std::chrono::steady_clock::now();
-    search_impl(ix, queries, nq, p, x, ids, dists, &c);
-    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double dt = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {  // the better of two timed runs
+      const auto t0 = std::chrono::steady_clock::now();
+      search_impl(ix, queries, nq, p, x, ids, dists, &c);
+      dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
     if (dt < best) {
       best = dt;
       pick = c;
     }
   };
-  // the band factor first (distinct band counts only), then the tile
-  uint32_t last_nb = 0;
-  for (uint32_t shift : {1u, 2u, 3u, 4u, 5u, 6u, 8u, 10u, 12u}) {
-    const uint32_t nb = katoann_hip::plan_bands(p.nprobe, shift);
-    if (nb == last_nb) continue;
-    last_nb = nb;
-    trial(TileConfig{1u, 0u, 1024u, shift});
+  // the bands first (seed rows, and band factors of distinct band counts),
+  // then the tile
+  const uint32_t s0 = std::max(64u, std::max(p.survivors, p.k));
+  for (uint32_t seed : {0u, 2 * s0, 4 * s0, 8 * s0, 16 * s0, 32 * s0}) {
+    uint32_t last_nb = 0;
+    for (uint32_t shift : {1u, 2u, 3u, 4u, 5u, 6u, 8u, 10u, 12u}) {
+      const uint32_t nb = katoann_hip::plan_bands(p.nprobe, shift, seed);
+      if (nb == last_nb || nb > kMaxBands) continue;
+      last_nb = nb;
+      trial(TileConfig{1u, 0u, 1024u, shift, seed});
+    }
   }
-  const uint32_t shift = pick.shift;
+  const uint32_t shift = pick.shift, seed = pick.seed;
   for (uint32_t variant : {0u, 1u, 2u, 3u})
-    for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift});
-  for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift});
+    for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift, seed});
+  for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift, seed});
   std::lock_guard<std::mutex> g(mix.tune_mu);
   mix.tuned[key] = pick;
 }
@@ -999,6 +1037,7 @@ int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_searc
   out[1] = c.variant;
   out[2] = c.qc;
   out[3] = c.shift;
+  out[4] = c.seed;
   KATOANN_CATCH
 }
 
@@ -1099,6 +1138,25 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor) {
   if (log2_factor < 1 || log2_factor > 12) throw std::invalid_argument("band factor must be 2^1 .. 2^12");
   std::lock_guard<std::mutex> g(h->mu);
   h->band_shift = log2_factor;
+  KATOANN_CATCH
+}
+
+int katoann_set_pool_allowance(katoann_index *h, uint32_t slots) {
+  KATOANN_TRY
+  std::lock_guard<std::mutex> g(h->mu);
+  h->pool_allowance = slots;
+  {
+    std::lock_guard<std::mutex> t(h->tune_mu);
+    h->allowance_seen.clear();
+  }
+  KATOANN_CATCH
+}
+
+int katoann_set_seed_rows(katoann_index *h, uint32_t rows) {
+  KATOANN_TRY
+  if (rows > (1u << 20)) throw std::invalid_argument("seed rows must be at most 2^20");
+  std::lock_guard<std::mutex> g(h->mu);
+  h->seed_rows = rows;
   KATOANN_CATCH
 }
 

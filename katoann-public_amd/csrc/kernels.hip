@@ -751,8 +751,12 @@ struct InvQuery {
   bool valid;
 };
 
+// Band-start keys of query q: kb[b] = key of the first probe of key band b
+// (b >= 1). Without a seed the key bands are the slot bands; with one, key
+// band 0 is the seed (slot bands 0 and 1) and key band b >= 1 is slot band
+// b + 1, starting at rank r0 + F (F^(b-1) - 1) / (F - 1) (r0 = seed[q]).
 __device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t q,
-                                     uint32_t B, uint32_t np, uint32_t nb, uint32_t shift) {
+                                     uint32_t B, uint32_t np, uint32_t nb, uint32_t shift, const uint32_t *seed) {
   InvQuery r;
   r.kend = 0;
   r.valid = q < B;
@@ -762,6 +766,17 @@ __device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_
   const uint32_t n = npq ? min(npq[q], np) : np;
   const size_t base = static_cast<size_t>(q) * np;
   r.kend = probe_key(probes, probe_d, base + n - 1);
+  if (seed) {
+    uint64_t sb = min(seed[q], n), step = 1ull << shift;
+#pragma unroll
+    for (int b = 1; b < INV_MAXB; ++b) {
+      if (static_cast<uint32_t>(b) + 1 >= nb) break;
+      if (sb < n) r.kb[b] = probe_key(probes, probe_d, base + sb);
+      sb += step;
+      step = min(step << shift, 1ull << 32);
+    }
+    return r;
+  }
   const uint32_t F1 = (1u << shift) - 1u;
 #pragma unroll
   for (int b = 1; b < INV_MAXB; ++b) {
@@ -770,6 +785,36 @@ __device__ inline InvQuery inv_query(const uint32_t *probes, const float *probe_
     if (sb < n) r.kb[b] = probe_key(probes, probe_d, base + sb);
   }
   return r;
+}
+
+// Seed ranks: per query, the fewest nearest lists whose sizes, each counted up
+// to R0 rows, reach R0 (all its probes if they do not). Probes are read eight
+// at a time so the loads of a step are independent.
+__global__ void k_seed_ranks(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq,
+                             const uint64_t *__restrict__ list_off, uint32_t B, uint32_t np, uint32_t seed,
+                             uint32_t *__restrict__ r0) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= B) return;
+  const uint32_t n = npq ? min(npq[q], np) : np;
+  const uint32_t *pr = probes + static_cast<size_t>(q) * np;
+  uint64_t cum = 0;
+  uint32_t r = 0;
+  while (r < n && cum < seed) {
+    uint64_t sz[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t c = r + u < n ? pr[r + u] : 0u;
+      sz[u] = r + u < n ? list_off[c + 1] - list_off[c] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (r < n && cum < seed) {
+        cum += min(sz[u], static_cast<uint64_t>(seed));
+        ++r;
+      }
+    }
+  }
+  r0[q] = r;
 }
 
 // One thread per list, walking a tile of INV_QT queries in ascending order
@@ -783,15 +828,16 @@ template <bool FILL, int NB>
 __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist, const uint32_t *__restrict__ probes,
                                                   const float *__restrict__ probe_d, const uint32_t *__restrict__ npq,
                                                   uint32_t B, uint32_t np, uint32_t nlist, uint32_t shift,
-                                                  uint32_t *__restrict__ part, const uint32_t *__restrict__ qoff,
-                                                  uint32_t *__restrict__ qlist) {
+                                                  const uint32_t *__restrict__ seed_r0, uint32_t seed,
+                                                  const uint64_t *__restrict__ list_off, uint32_t *__restrict__ part,
+                                                  const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist) {
   constexpr uint32_t nb = NB;
   __shared__ uint64_t s_kend[INV_QT];
   __shared__ uint64_t s_kb[NB][INV_QT];
   const uint32_t qt = blockIdx.y, q0 = qt * INV_QT, c = blockIdx.x * INV_LB + threadIdx.x;
   const uint32_t nslots = nb * nlist;
   if (threadIdx.x < INV_QT) {
-    const InvQuery r = inv_query(probes, probe_d, npq, q0 + threadIdx.x, B, np, nb, shift);
+    const InvQuery r = inv_query(probes, probe_d, npq, q0 + threadIdx.x, B, np, nb, shift, seed ? seed_r0 : nullptr);
     s_kend[threadIdx.x] = r.kend;
 #pragma unroll
     for (int b = 0; b < NB; ++b) s_kb[b][threadIdx.x] = r.kb[b];
@@ -799,6 +845,8 @@ __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist
   __syncthreads();
   if (c >= nlist) return;
   const uint32_t nq = min(static_cast<uint32_t>(INV_QT), B - q0);
+  // with a seed, key band 0 also feeds slot band 1 when the list has rows past R0
+  const bool rest = seed && list_off[c + 1] - list_off[c] > seed;
   uint32_t acc[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -820,9 +868,10 @@ __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist
       uint32_t band = 0;
 #pragma unroll
       for (int b = 1; b < NB; ++b) band += k >= s_kb[b][i] ? 1u : 0u;
+      if (seed && band) ++band;  // key band b >= 1 -> slot band b + 1
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        if (band == static_cast<uint32_t>(b)) {
+        if (band == static_cast<uint32_t>(b) || (b == 1 && band == 0 && rest)) {
           if (FILL) qlist[acc[b]] = q0 + i;
           ++acc[b];
         }
@@ -864,8 +913,8 @@ __device__ inline uint64_t block_exscan(uint64_t v, uint64_t *tmp, uint64_t *tot
 // per block of 1024 slots: their sums.
 __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t qtiles, uint32_t nslots,
                                                       uint32_t nlist, const uint64_t *__restrict__ list_off,
-                                                      uint32_t qc, uint32_t rt, uint32_t *gcnt, uint32_t *nit,
-                                                      uint32_t *bsum) {
+                                                      uint32_t qc, uint32_t rt, uint32_t seed, uint32_t *gcnt,
+                                                      uint32_t *nit, uint32_t *bsum) {
   __shared__ uint64_t tmp[17];
   const uint32_t sl = blockIdx.x * 1024 + threadIdx.x;
   uint32_t g = 0, it = 0;
@@ -877,7 +926,9 @@ __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t q
       g += v;
     }
     const uint32_t c = sl % nlist;
-    const uint32_t n = static_cast<uint32_t>(list_off[c + 1] - list_off[c]);
+    uint64_t first, end;
+    katoann_hip::slot_rows(sl / nlist, list_off[c + 1] - list_off[c], seed, first, end);
+    const uint32_t n = static_cast<uint32_t>(end - first);
     it = (g && n) ? ((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0u;
     gcnt[sl] = g;
     nit[sl] = it;
@@ -1066,23 +1117,26 @@ void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, u
   hipLaunchKernelGGL(k_select, dim3(B), dim3(256), sm, s, dist, B, nlist, np, probes, probe_d);
 }
 
-uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift) { return plan_bands(np, shift) * nlist; }
+uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) {
+  return plan_bands(np, shift, seed) * nlist;
+}
 
-size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift) {
-  const size_t nslots = plan_slots(nlist, np, shift);
+size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) {
+  const size_t nslots = plan_slots(nlist, np, shift, seed);
   const size_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
-  return qtiles * nslots + nslots + 2 * nblk;
+  return qtiles * nslots + nslots + 2 * nblk + B;
 }
 
 template <bool FILL>
 static void inv_pass_launch(uint32_t nb, dim3 tiles, hipStream_t s, const float *dist, const uint32_t *probes,
                             const float *probe_d, const uint32_t *npq, uint32_t B, uint32_t np, uint32_t nlist,
-                            uint32_t shift, uint32_t *part, const uint32_t *qoff, uint32_t *qlist) {
+                            uint32_t shift, const uint32_t *seed_r0, uint32_t seed, const uint64_t *list_off,
+                            uint32_t *part, const uint32_t *qoff, uint32_t *qlist) {
   switch (nb) {
 #define NB_CASE(n)                                                                                              \
   case n:                                                                                                       \
     hipLaunchKernelGGL((k_inv_pass<FILL, n>), tiles, dim3(256), 0, s, dist, probes, probe_d, npq, B, np, nlist, \
-                       shift, part, qoff, qlist);                                                               \
+                       shift, seed_r0, seed, list_off, part, qoff, qlist);                                      \
     break;
     NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
     NB_CASE(9) NB_CASE(10) NB_CASE(11) NB_CASE(12) NB_CASE(13) NB_CASE(14) NB_CASE(15) NB_CASE(16)
@@ -1096,17 +1150,23 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                  uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
                  uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
                  uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
-                 uint32_t nstate, uint32_t shift, hipStream_t s) {
-  const uint32_t nb = plan_bands(np, shift), nslots = nb * nlist;
+                 uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
+  const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
   const uint32_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
+  uint32_t *seed_r0 = bsum + 2 * nblk;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
-  inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, part, nullptr, nullptr);
-  hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, gcnt,
-                     nit, bsum);
+  if (seed)
+    hipLaunchKernelGGL(k_seed_ranks, dim3((B + 255) / 256), dim3(256), 0, s, probes, npq, list_off, B, np, seed,
+                       seed_r0);
+  inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
+                         nullptr, nullptr);
+  hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, seed,
+                     gcnt, nit, bsum);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
-  inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, part, qoff, qlist);
+  inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
+                        qoff, qlist);
   hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr,
                      lbound);
 }

@@ -23,28 +23,53 @@ void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, u
 size_t select_smem(uint32_t nlist, uint32_t np);
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
                    hipStream_t s);
-// Probe-rank bands for band factor F = 2^shift: band b holds ranks
-// [(F^b - 1) / (F - 1), (F^(b+1) - 1) / (F - 1)); the work plan has one
+// Probe-rank bands for band factor F = 2^shift; the work plan has one
 // (band, list) slot per band and list.
-inline uint32_t plan_bands(uint32_t np, uint32_t shift) {
-  const uint32_t v = ((1u << shift) - 1u) * (np - 1u) + 1u;
-  return (31u - static_cast<uint32_t>(__builtin_clz(v))) / shift + 1u;
+// Without a seed (seed = 0), band b holds ranks [(F^b - 1) / (F - 1),
+// (F^(b+1) - 1) / (F - 1)).
+// With a seed of R0 rows, band 0 holds a query's nearest lists up to R0 rows
+// (the first r0 ranks: the fewest whose lists, each counted up to R0 rows,
+// reach R0), each list's first R0 rows only; band 1 the remaining rows of
+// those lists; band 2 + j ranks [r0 + F (F^j - 1) / (F - 1),
+// r0 + F (F^(j+1) - 1) / (F - 1)). Every query with R0 >= S then has a
+// threshold before its wider bands, and no pool takes more than 2 R0 keys
+// without one.
+inline uint32_t plan_bands(uint32_t np, uint32_t shift, uint32_t seed = 0) {
+  if (!seed) {
+    const uint32_t v = ((1u << shift) - 1u) * (np - 1u) + 1u;
+    return (31u - static_cast<uint32_t>(__builtin_clz(v))) / shift + 1u;
+  }
+  uint32_t nb = 2;  // the regular bands for the smallest r0 (1)
+  uint64_t s = 1, step = 1ull << shift;
+  while (s < np) {
+    ++nb;
+    s += step;
+    step = step << shift > (1ull << 32) ? (1ull << 32) : step << shift;
+  }
+  return nb;
 }
-uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift);
+uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed = 0);
 // 32-bit words of plan scratch for a batch (tile partial counts, item counts,
-// block sums).
-size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift);
+// block sums, seed ranks).
+size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed = 0);
 // Inversion and work plan from the coarse matrix and the sorted probes.
 void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
                  uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
                  uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
                  uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
-                 uint32_t nstate, uint32_t shift, hipStream_t s);
+                 uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
-                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint4 *items,
-                  hipStream_t s);
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
+                  uint4 *items, hipStream_t s);
+// Rows [first, end) of list c's n rows that slot band `band` ranks.
+__host__ __device__ inline void slot_rows(uint32_t band, uint64_t n, uint32_t seed, uint64_t &first, uint64_t &end) {
+  first = 0;
+  end = n;
+  if (seed && band == 0) end = n < seed ? n : seed;
+  if (seed && band == 1) first = n < seed ? n : seed;
+}
 bool launch_rank_band(uint32_t band, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, uint32_t nlist,
                       uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const int32_t *lbound,
                       const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,

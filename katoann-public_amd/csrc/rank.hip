@@ -88,7 +88,8 @@ struct RankShape {
 // (band, list) slot in item_off.
 __global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, uint32_t nlist,
                         const uint64_t *__restrict__ list_off, const uint32_t *__restrict__ gcnt,
-                        const uint32_t *__restrict__ qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint4 *items) {
+                        const uint32_t *__restrict__ qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
+                        uint4 *items) {
   const uint32_t it = blockIdx.x * 256 + threadIdx.x;
   const uint32_t total = item_off[nslots];
   if (it >= total || it >= cap) return;
@@ -99,15 +100,17 @@ __global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, 
     else hi = mid;
   }
   const uint32_t slot = lo, c = slot % nlist;
-  const uint64_t n = list_off[c + 1] - list_off[c];
+  uint64_t first, end;
+  katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
+  const uint64_t n = end - first;
   const uint32_t tiles = static_cast<uint32_t>((n + rt - 1) / rt);
   const uint32_t local = it - item_off[slot];
   const uint32_t qch = local / tiles, rtile = local % tiles;
   uint4 d;
   d.x = qoff[slot] + qch * qc;
   d.y = min(qc, gcnt[slot] - qch * qc);
-  d.z = static_cast<uint32_t>(list_off[c] + static_cast<uint64_t>(rtile) * rt);
-  d.w = static_cast<uint32_t>(list_off[c] + min(n, static_cast<uint64_t>(rtile + 1) * rt));
+  d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(rtile) * rt);
+  d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(rtile + 1) * rt));
   items[it] = d;
 }
 
@@ -1243,10 +1246,10 @@ uint32_t rank_queries_per_item(uint32_t P) {
 }
 
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
-                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint4 *items,
-                  hipStream_t s) {
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
+                  uint4 *items, hipStream_t s) {
   hipLaunchKernelGGL(k_items, dim3((cap + 255) / 256), dim3(256), 0, s, item_off, nslots, nlist, list_off, gcnt, qoff,
-                     qc, rt, cap, items);
+                     qc, rt, cap, seed, items);
 }
 
 template <int KC>

@@ -114,6 +114,8 @@ def lib():
         L.katoann_last_work.argtypes = [P, P, i32]
         L.katoann_set_rank_rows.argtypes = [P, u32]
         L.katoann_set_band_factor.argtypes = [P, u32]
+        L.katoann_set_seed_rows.argtypes = [P, u32]
+        L.katoann_set_pool_allowance.argtypes = [P, u32]
         L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
         L.katoann_set_graphs.argtypes = [P, i32]
         L.katoann_set_autotune.argtypes = [P, i32]
@@ -373,6 +375,17 @@ class Index:
         """Probe-rank bands of the ranking pass grow by 2**log2_factor (1..12)."""
         _check(lib().katoann_set_band_factor(self._h, log2_factor))
 
+    def set_seed_rows(self, rows: int):
+        """Seed band of the ranking pass: each query's nearest lists up to
+        `rows` rows (at most `rows` of each) before the probe-rank bands, so
+        every query has a threshold early; 0 = no seed band."""
+        _check(lib().katoann_set_seed_rows(self._h, rows))
+
+    def set_pool_allowance(self, slots: int):
+        """Survivor pool room per query beyond S (0 = sized per search); a band
+        that keeps more reruns the batch with 4x the room."""
+        _check(lib().katoann_set_pool_allowance(self._h, slots))
+
     def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 512):
         """Ranking kernel: 0 = block items (queries staged into LDS), 1 = wave
         items (rows held in registers, queries streamed), 2 = block items with a
@@ -387,12 +400,13 @@ class Index:
         _check(lib().katoann_set_autotune(self._h, int(on)))
 
     def tile_config(self, nq, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold"):
-        """The tile (kernel, variant, queries per item, log2 band factor) a
-        search of this shape runs with."""
-        out = np.zeros(4, np.uint32)
+        """The tile (kernel, variant, queries per item, log2 band factor, seed
+        rows) a search of this shape runs with."""
+        out = np.zeros(5, np.uint32)
         prm = self._params(k, nprobe, ranking, survivors, selection)
         _check(lib().katoann_tile_config(self._h, nq, C.byref(prm), _ptr(out)))
-        return dict(kernel=int(out[0]), variant=int(out[1]), queries_per_item=int(out[2]), log2_band_factor=int(out[3]))
+        return dict(kernel=int(out[0]), variant=int(out[1]), queries_per_item=int(out[2]), log2_band_factor=int(out[3]),
+                    seed_rows=int(out[4]))
 
     def set_graphs(self, on: bool = True):
         """Replay each search's kernels from a captured hipGraph (default on)."""

@@ -98,3 +98,24 @@ def test_search_argument_checks():
 def test_server_needs_a_loaded_index():
     with pytest.raises(RuntimeError, match="not loaded"):
         katoann.Server(katoann.Index())
+
+
+def test_tuning_knob_ranges():
+    """The ranking knobs reject values outside their ranges before any GPU work."""
+    ix = katoann.Index()
+    with pytest.raises(ValueError, match="band factor"):
+        ix.set_band_factor(13)
+    with pytest.raises(ValueError, match="band factor"):
+        ix.set_band_factor(0)
+    with pytest.raises(ValueError, match="rank kernel"):
+        ix.set_rank_kernel(2, 2)
+    with pytest.raises(ValueError, match="rank kernel"):
+        ix.set_rank_kernel(1, 4)
+    with pytest.raises(ValueError, match="queries per ranking item"):
+        ix.set_rank_kernel(1, 0, 0)
+    with pytest.raises(ValueError, match="seed rows"):
+        ix.set_seed_rows(1 << 21)
+    ix.set_band_factor(12)
+    ix.set_rank_kernel(1, 3, 4096)
+    ix.set_seed_rows(0)
+    ix.set_pool_allowance(0)

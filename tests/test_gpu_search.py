@@ -106,6 +106,40 @@ def test_per_query_nprobe(mixture_index):
     check_against_oracle(ix, ox, q, 10, 32, 32, npq=npq)
 
 
+@pytest.mark.parametrize("seed,shift", [(32, 2), (64, 1), (500, 3), (5000, 12)])
+def test_seed_band_vs_oracle(mixture_index, wide_index, seed, shift):
+    """With a seed band (a query's nearest lists up to `seed` rows first, then
+    the rest of them, then probe-rank bands), survivors and results are the
+    oracle's, with per-query nprobe too; a seed below S is raised to S."""
+    rng = np.random.default_rng(seed)
+    for name, (prefix, base, q) in (("mixture", mixture_index), ("wide", wide_index)):
+        ix = load(prefix)
+        ix.set_seed_rows(seed)
+        ix.set_band_factor(shift)
+        ox = _oracle(prefix)
+        for (k, nprobe, S) in ((10, 16, 32), (10, 4, 128), (100, 8, 100)):
+            print("seed", seed, shift, name, k, nprobe, S, flush=True)
+            check_against_oracle(ix, ox, q[:300], k, min(nprobe, ix.nlist), S)
+        npq = rng.integers(1, 33, 300).astype(np.uint32)
+        print("seed per-query nprobe", seed, shift, name, flush=True)
+        check_against_oracle(ix, ox, q[:300], 10, min(32, ix.nlist), 32, npq=npq)
+
+
+def test_pool_overflow_reruns(mixture_index):
+    """A pool too small for a band's kept keys: the batch runs again with more
+    room (and later searches of the shape start from it); results stay the
+    oracle's, with and without a seed band."""
+    prefix, base, q = mixture_index
+    ox = _oracle(prefix)
+    for seed in (0, 64):
+        ix = load(prefix)
+        ix.set_seed_rows(seed)
+        ix.set_pool_allowance(8)
+        for rep in range(2):
+            print("overflow rerun", seed, rep, flush=True)
+            check_against_oracle(ix, ox, q[:300], 10, 16, 32)
+
+
 def test_ragged_and_single_batches(mixture_index):
     """Each query's result is the same alone, in a ragged batch and in the full batch."""
     prefix, base, q = mixture_index
@@ -217,10 +251,11 @@ def test_rank_kernel_choice_is_invisible(mixture_index):
     ix = load(prefix)
     ref = None
     for (kernel, variant, qc) in RANK_KERNELS:
-        for shift in (1, 2, 4, 7):
+        for shift, seed in ((1, 0), (2, 0), (4, 0), (7, 0), (2, 64), (3, 300), (12, 100)):
             ix.set_rank_kernel(kernel, variant, qc)
             ix.set_band_factor(shift)
-            print("choice", kernel, variant, qc, shift, flush=True)
+            ix.set_seed_rows(seed)
+            print("choice", kernel, variant, qc, shift, seed, flush=True)
             i, d, _ = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)
             if ref is None:
                 ref = (i, d)
@@ -262,7 +297,7 @@ def test_autotune_keeps_results(mixture_index):
         i, d, _ = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)
         assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
     tuned = ix.tile_config(q.shape[0], k=10, nprobe=64, survivors=64)
-    assert tuned["kernel"] in (1, 2) and tuned["queries_per_item"] in (256, 512, 1024)
+    assert tuned["kernel"] in (1, 2) and tuned["queries_per_item"] in (256, 512, 1024, 4096)
     print("default tile", default, "tuned", tuned, flush=True)
 
 
