@@ -324,28 +324,37 @@ __global__ __launch_bounds__(256, 1) void k_coarse(const int8_t *__restrict__ qh
     for (int b = 0; b < 4; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
   const uint32_t nk = dc / 64;
   fetch(0);
+  // norms of the lane's output rows and columns, loaded behind the main loop
+  int64_t qn[4][4], cn[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) qn[a][r] = qnorm[min(q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r, B - 1)];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) cn[t] = cnorm[min(c0 + 64 * wc + 16 * t + (lane & 15), nlist - 1)];
   stash(0);
   __syncthreads();
   for (uint32_t k = 0; k < nk; ++k) {
     const int b = k & 1;
     if (k + 1 < nk) fetch((k + 1) * 64);
     const int8_t *S = st[b];
-    v4i bh[4], bl[4];
+    // all of the step's fragments first: one wait for the LDS reads
+    v4i bh[4], bl[4], ah[4], al[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       bh[t] = *reinterpret_cast<const v4i *>(S + 2 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
       bl[t] = *reinterpret_cast<const v4i *>(S + 3 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
+      ah[t] = *reinterpret_cast<const v4i *>(S + (4 * wq + t) * 1024 + lane * 16);
+      al[t] = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + t) * 1024 + lane * 16);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const v4i ah = *reinterpret_cast<const v4i *>(S + (4 * wq + a) * 1024 + lane * 16);
-      const v4i al = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + a) * 1024 + lane * 16);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        hh[a][t] = mfma(ah, bh[t], hh[a][t]);
-        mid[a][t] = mfma(ah, bl[t], mid[a][t]);
-        mid[a][t] = mfma(al, bh[t], mid[a][t]);
-        ll[a][t] = mfma(al, bl[t], ll[a][t]);
+        hh[a][t] = mfma(ah[a], bh[t], hh[a][t]);
+        mid[a][t] = mfma(ah[a], bl[t], mid[a][t]);
+        mid[a][t] = mfma(al[a], bh[t], mid[a][t]);
+        ll[a][t] = mfma(al[a], bl[t], ll[a][t]);
       }
     }
     if (k + 1 < nk) {
@@ -357,17 +366,14 @@ __global__ __launch_bounds__(256, 1) void k_coarse(const int8_t *__restrict__ qh
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const uint32_t c = c0 + 64 * wc + 16 * t + (lane & 15);
-    if (c >= nlist) continue;
-    const int64_t cn = cnorm[c];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const uint32_t q = q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r;
-        if (q >= B) continue;
         const int64_t dot = static_cast<int64_t>(hh[a][t][r]) * 65536 + static_cast<int64_t>(mid[a][t][r]) * 256 +
                             static_cast<int64_t>(ll[a][t][r]);
-        dist[static_cast<size_t>(q) * nlist + c] = fscale(qnorm[q] + cn - 2 * dot, step_sq);
+        if (q < B && c < nlist) dist[static_cast<size_t>(q) * nlist + c] = fscale(qn[a][r] + cn[t] - 2 * dot, step_sq);
       }
   }
 }
