@@ -588,11 +588,13 @@ __device__ __forceinline__ uint32_t count_below(const uint64_t *a, uint32_t n, u
   return lo;
 }
 
+constexpr uint32_t SEL_FAST = 1024;  // candidates of the small-np path at most
+
 template <int V>
 __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
                                                     uint32_t np, uint32_t *__restrict__ probes,
                                                     float *__restrict__ probe_d) {
-  extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x M
+  extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x max(M, SEL_FAST if np <= 256)
   __shared__ uint32_t hist[4][SEL_BINS];
   __shared__ uint32_t tmp[8], sh_digit, sh_rank;
   const uint32_t q = blockIdx.x;
@@ -636,6 +638,79 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
   vmin = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
   vmax = max(max(tmp[4], tmp[5]), max(tmp[6], tmp[7]));
   __syncthreads();
+  // the kept keys, sorted: runs of W keys by the waves, then merge levels;
+  // the first np go out
+  auto sort_and_emit = [&](uint32_t M) {
+    const uint32_t W = M >= 256 ? M / 4 : M, nruns = M / W;
+    sort_runs(keys, W, nruns);
+    __syncthreads();
+    uint64_t *src = keys, *dst = keys + M;
+    for (uint32_t L = W; L < M; L <<= 1) {
+      for (uint32_t e = tid; e < M; e += 256) {
+        const uint32_t pair = e / (2 * L) * (2 * L), off = e - pair;
+        const uint64_t x = src[e];
+        const bool inA = off < L;
+        const uint64_t *other = src + pair + (inA ? L : 0);
+        const uint32_t pos = (inA ? off : off - L) + count_below(other, L, x, !inA);
+        dst[pair + pos] = x;
+      }
+      __syncthreads();
+      uint64_t *t = src;
+      src = dst;
+      dst = t;
+    }
+    for (uint32_t p = tid; p < np; p += 256) {
+      const uint64_t k = src[p];
+      probes[static_cast<size_t>(q) * np + p] = static_cast<uint32_t>(k);
+      probe_d[static_cast<size_t>(q) * np + p] = __uint_as_float(static_cast<uint32_t>(k >> 32));
+    }
+  };
+  if (np <= 256) {
+    // Each wave sorts its threads' minimum keys (one per lane) and takes the
+    // ceil(np / 4)-th; at least np keys lie at or below the largest of the
+    // four, and for rows without long runs of small values in few threads
+    // not many more: those candidates hold the answer. Past SEL_FAST
+    // candidates the radix path below runs instead.
+    uint64_t mk = kNoKey;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+      if (c0 + i < nlist) mk = umin64(mk, (static_cast<uint64_t>(v[i]) << 32) | (c0 + i));
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const uint64_t y = static_cast<uint64_t>(__shfl_xor(static_cast<long long>(mk), j));
+        const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+        mk = lower == up ? umin64(mk, y) : umax64(mk, y);
+      }
+    const uint64_t wk = static_cast<uint64_t>(__shfl(static_cast<long long>(mk), static_cast<int>((np + 3) / 4 - 1)));
+    if (lane == 0) keys[w] = wk;
+    __syncthreads();
+    const uint64_t tk = umax64(umax64(keys[0], keys[1]), umax64(keys[2], keys[3]));
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+      if (c0 + i < nlist) cnt += ((static_cast<uint64_t>(v[i]) << 32) | (c0 + i)) <= tk ? 1u : 0u;
+    uint32_t off = block_exscan256(cnt, tmp);
+    if (tid == 255) sh_rank = off + cnt;
+    __syncthreads();
+    const uint32_t total = sh_rank;
+    if (total <= SEL_FAST) {
+      uint32_t C = 64;
+      while (C < total) C <<= 1;
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (c0 + i < nlist) {
+          const uint64_t key = (static_cast<uint64_t>(v[i]) << 32) | (c0 + i);
+          if (key <= tk) keys[off++] = key;
+        }
+      for (uint32_t p = total + tid; p < C; p += 256) keys[p] = kNoKey;
+      __syncthreads();
+      sort_and_emit(C);
+      return;
+    }
+    __syncthreads();
+  }
   // radix select of the np-th smallest on r = v - vmin, 11-bit digits
   const uint32_t range = vmax - vmin;
   const int nbits = range ? 32 - __clz(range) : 0;
@@ -699,30 +774,7 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
     }
   for (uint32_t p = np + tid; p < M; p += 256) keys[p] = kNoKey;
   __syncthreads();
-  // sort: runs of W keys by the waves, then merge levels
-  const uint32_t W = M >= 256 ? M / 4 : M, nruns = M / W;
-  sort_runs(keys, W, nruns);
-  __syncthreads();
-  uint64_t *src = keys, *dst = keys + M;
-  for (uint32_t L = W; L < M; L <<= 1) {
-    for (uint32_t e = tid; e < M; e += 256) {
-      const uint32_t pair = e / (2 * L) * (2 * L), off = e - pair;
-      const uint64_t x = src[e];
-      const bool inA = off < L;
-      const uint64_t *other = src + pair + (inA ? L : 0);
-      const uint32_t pos = (inA ? off : off - L) + count_below(other, L, x, !inA);
-      dst[pair + pos] = x;
-    }
-    __syncthreads();
-    uint64_t *t = src;
-    src = dst;
-    dst = t;
-  }
-  for (uint32_t p = tid; p < np; p += 256) {
-    const uint64_t k = src[p];
-    probes[static_cast<size_t>(q) * np + p] = static_cast<uint32_t>(k);
-    probe_d[static_cast<size_t>(q) * np + p] = __uint_as_float(static_cast<uint32_t>(k >> 32));
-  }
+  sort_and_emit(M);
 }
 
 // --------------------------------------------------------------------------
@@ -1104,7 +1156,8 @@ static void select_reg_launch(const float *dist, uint32_t B, uint32_t nlist, uin
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(2 * SEL_MAXNP * 8));
     attr = true;
   }
-  hipLaunchKernelGGL(k_select_reg<V>, dim3(B), dim3(256), 2 * M * 8, s, dist, B, nlist, np, probes, probe_d);
+  const uint32_t words = np <= 256 && M < SEL_FAST ? 2 * SEL_FAST : 2 * M;  // the sort's two buffers
+  hipLaunchKernelGGL(k_select_reg<V>, dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
 }
 
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
