@@ -172,14 +172,14 @@ int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n);
 int katoann_set_rank_rows(katoann_index *h, uint32_t rows);
 /* Probe-rank bands of the ranking pass: band b holds ranks [(F^b - 1) / (F - 1),
  * (F^(b+1) - 1) / (F - 1)) with F = 2^log2_factor (1..12); the queries' thresholds
- * tighten between bands. Default 2 (F = 4). */
+ * tighten between bands. Default 5 (F = 32). */
 int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
 /* Seed band (no reference counterpart): with rows = R0 > 0 the ranking pass
  * first ranks each query's nearest lists up to R0 rows in all (at most the
  * first R0 rows of each; R0 is raised to max(survivors, k)), then the rest of
  * those lists, then probe-rank bands of factor F from the next rank. Every
- * query thus has a threshold before its wider bands. 0 (default) = bands from
- * rank 0. Results do not depend on it. */
+ * query thus has a threshold before its wider bands. 0 = bands from rank 0.
+ * Default 1024. Results do not depend on it. */
 int katoann_set_seed_rows(katoann_index *h, uint32_t rows);
 /* Survivor pool room per query beyond S for one band's kept keys (no
  * reference counterpart): 0 (default) sizes it per search. A band that keeps
@@ -193,7 +193,7 @@ int katoann_set_pool_allowance(katoann_index *h, uint32_t slots);
  * variant 0 or 1 picks the larger or smaller T; variant 2 (kernel 1 only) a
  * smaller T with query codes loaded two fragments ahead instead of one;
  * variant 3 (kernel 1 only) a larger T at one wave per SIMD.
- * Default: 1, variant 0, 512. */
+ * Default: 1, variant 0, 1024. */
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item);
 /* Graph replay (no reference counterpart): each search context captures its
  * kernel sequence (normalisation through top-k) into a hipGraph once and

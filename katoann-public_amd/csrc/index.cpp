@@ -128,10 +128,10 @@ struct katoann_index {
   std::vector<uint64_t> list_off;
   uint64_t nrows = 0;
   uint32_t rank_rows = 1024;
-  uint32_t band_shift = 2;  // probe-rank bands grow by 2^band_shift
-  uint32_t seed_rows = 0;   // rows of the seed band (0: none)
+  uint32_t band_shift = 5;     // probe-rank bands grow by 2^band_shift
+  uint32_t seed_rows = 1024;  // rows of the seed band (0: none)
   uint32_t pool_allowance = 0;  // pool slots per query beyond S (0: sized per search)
-  uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 512;
+  uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 1024;
   std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
   std::atomic<bool> autotune{false};
   std::mutex tune_mu;

@@ -386,7 +386,7 @@ class Index:
         that keeps more reruns the batch with 4x the room."""
         _check(lib().katoann_set_pool_allowance(self._h, slots))
 
-    def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 512):
+    def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 1024):
         """Ranking kernel: 0 = block items (queries staged into LDS), 1 = wave
         items (rows held in registers, queries streamed), 2 = block items with a
         shared query ring; variant 0/1 = larger or smaller row tile per wave,
