@@ -257,18 +257,22 @@ MFMA_I8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
 
 
 def profiled_traffic(r, args):
-    """HBM bytes per search of the ranking pass from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary, when it was taken on this configuration."""
-    path = os.path.join(ROOT, "profiles", "k_rank_wave_1Mx1024_nlist4096_b4096_np1024_S128.md")
-    want = "spectrum %dx%d nlist %d nprobe %d survivors %d batch %d" % (
-        args.n, args.dim, args.nlist, r["nprobe"], r["survivors"], args.nq)
-    try:
-        lines = dict(l.split(": ", 1) for l in open(path).read().splitlines() if l.startswith("traffic_"))
-    except OSError:
-        return None
-    if args.workload != "spectrum" or lines.get("traffic_config", "").strip() != want:
-        return None
-    return float(lines["traffic_bytes_per_search"])
+    """HBM bytes per search of the ranking pass from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/*.md) taken on this workload,
+    search shape and ranking tile."""
+    t = r.get("tile") or {}
+    want = "%s %dx%d nlist %d nprobe %d survivors %d batch %d tile %d,%d,%d shift %d seed %d" % (
+        args.workload, args.n, args.dim, args.nlist, r["nprobe"], r["survivors"], args.nq, t.get("kernel", -1),
+        t.get("variant", -1), t.get("queries_per_item", -1), t.get("log2_band_factor", -1), t.get("seed_rows", -1))
+    pdir = os.path.join(ROOT, "profiles")
+    for fn in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not fn.endswith(".md"):
+            continue
+        lines = dict(l.split(": ", 1) for l in open(os.path.join(pdir, fn)).read().splitlines()
+                     if l.startswith("traffic_"))
+        if lines.get("traffic_config", "").strip() == want:
+            return float(lines["traffic_bytes_per_search"])
+    return None
 
 
 def roofline(r):
