@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=1024)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-train", type=int, default=50000, help="training queries of the learned nprobe")
-    ap.add_argument("--learned-nmax", type=int, default=2048)
+    ap.add_argument("--learned-nmax", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
