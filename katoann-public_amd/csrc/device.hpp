@@ -249,4 +249,85 @@ __device__ inline uint64_t kth_key(const uint64_t (&e)[CH], uint32_t S) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Block-wide exclusive scan of one value per thread (256 threads); tmp holds
+// 4 words.
+__device__ inline uint32_t block_exscan256(uint32_t v, uint32_t *tmp) {
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[w] = x;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int k = 0; k < w; ++k) base += tmp[k];
+  __syncthreads();
+  return base + x - v;
+}
+
+template <int W>
+__device__ inline void wave_sort_run(uint64_t *run) {
+  const int lane = lane_id();
+  TopList<W> t;
+#pragma unroll
+  for (int r = 0; r < W / 64; ++r) t.e[r] = run[64 * r + lane];
+  list_sort<W>(t);
+#pragma unroll
+  for (int r = 0; r < W / 64; ++r) run[64 * r + lane] = t.e[r];
+}
+
+__device__ inline void sort_runs(uint64_t *buf, uint32_t W, uint32_t nruns) {
+  const int w = threadIdx.x >> 6;
+  if (static_cast<uint32_t>(w) >= nruns) return;
+  uint64_t *run = buf + static_cast<size_t>(w) * W;
+  switch (W) {
+    case 64: wave_sort_run<64>(run); break;
+    case 128: wave_sort_run<128>(run); break;
+    case 256: wave_sort_run<256>(run); break;
+    case 512: wave_sort_run<512>(run); break;
+    default: wave_sort_run<1024>(run); break;
+  }
+}
+
+// number of keys in sorted run a[0, n) below x (upper: at or below x)
+__device__ __forceinline__ uint32_t count_below(const uint64_t *a, uint32_t n, uint64_t x, bool upper) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (upper ? a[mid] <= x : a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+
+// Sorts keys[0, M) ascending (M a power of two, 64..4096, 256 threads), with
+// keys[M, 2 M) as the second buffer: runs of M / 4 (or M) keys sorted by the
+// waves in registers, then merge levels by rank (merge path); returns the
+// buffer that holds the result.
+__device__ inline uint64_t *block_sort_keys(uint64_t *keys, uint32_t M) {
+  const int tid = threadIdx.x;
+  const uint32_t W = M >= 256 ? M / 4 : M, nruns = M / W;
+  sort_runs(keys, W, nruns);
+  __syncthreads();
+  uint64_t *src = keys, *dst = keys + M;
+  for (uint32_t L = W; L < M; L <<= 1) {
+    for (uint32_t e = tid; e < M; e += 256) {
+      const uint32_t pair = e / (2 * L) * (2 * L), off = e - pair;
+      const uint64_t x = src[e];
+      const bool inA = off < L;
+      const uint64_t *other = src + pair + (inA ? L : 0);
+      const uint32_t pos = (inA ? off : off - L) + count_below(other, L, x, !inA);
+      dst[pair + pos] = x;
+    }
+    __syncthreads();
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
 }  // namespace katoann_dev

@@ -976,14 +976,90 @@ __device__ inline void list_merge_from(TopList<SC> &t, const uint64_t *u, uint32
     if (static_cast<uint32_t>(lane + 64 * r) >= S) t.e[r] = kNoKey;
 }
 
+constexpr uint32_t UPD_CAP = 1024;  // candidates of the selection path at most
+constexpr int UPD_R = 16;           // pool keys per thread on the selection path (pools up to 4,096)
+
+// Pools large enough for the selection path (k_update_sel); k_update merges
+// the others (small pools merge faster: the batches past the first S keys are
+// mostly dropped by the cap before any sorting).
+__device__ __forceinline__ bool update_by_selection(uint32_t S, uint32_t n) {
+  return S <= 256 && n > max(4 * S, 1024u) && n <= 256u * UPD_R;
+}
+
+// Selection path: at least S keys lie at or below the largest of the waves'
+// ceil(S / 4)-th smallest thread minimum (each wave sorts its 64 threads'
+// minima, one per lane), and for pools in no particular order not many more;
+// those candidates, sorted by the four waves, give the S smallest. A query
+// with more than UPD_CAP candidates is left to k_update (run after it).
+// Each thread's keys are loaded at once (independent loads) into registers;
+// real keys are below kNoKey.
+__global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
+  __shared__ uint64_t skeys[2 * UPD_CAP];
+  __shared__ uint64_t swk[4];
+  __shared__ uint32_t stmp[4], stotal;
+  const int lane = lane_id(), w = threadIdx.x >> 6, tid = threadIdx.x;
+  const uint32_t q = blockIdx.x;
+  const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride);
+  if (n == a.pool_sorted[q] || !update_by_selection(a.S, n)) return;
+  uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
+  uint64_t v[UPD_R];
+#pragma unroll
+  for (int r = 0; r < UPD_R; ++r) {
+    const uint32_t i = tid + 256u * r;
+    v[r] = i < n ? pq[i] : kNoKey;
+  }
+  uint64_t mk = kNoKey;
+#pragma unroll
+  for (int r = 0; r < UPD_R; ++r) mk = umin64(mk, v[r]);
+  mk = wave_sort64(mk);
+  const uint64_t wk = static_cast<uint64_t>(__shfl(static_cast<long long>(mk), static_cast<int>((a.S + 3) / 4 - 1)));
+  if (lane == 0) swk[w] = wk;
+  __syncthreads();
+  const uint64_t tk = umax64(umax64(swk[0], swk[1]), umax64(swk[2], swk[3]));
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r = 0; r < UPD_R; ++r) cnt += v[r] != kNoKey && v[r] <= tk ? 1u : 0u;
+  uint32_t off = block_exscan256(cnt, stmp);
+  if (tid == 255) stotal = off + cnt;
+  __syncthreads();
+  const uint32_t total = stotal;
+  if (total > UPD_CAP) {
+    // left for k_update: mark the query by a flag bit in pool_sorted
+    if (tid == 0) a.pool_sorted[q] |= 0x80000000u;
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < UPD_R; ++r)
+    if (v[r] != kNoKey && v[r] <= tk) skeys[off++] = v[r];
+  uint32_t M = 64;
+  while (M < total) M <<= 1;
+  for (uint32_t i = total + tid; i < M; i += 256) skeys[i] = kNoKey;
+  __syncthreads();
+  const uint64_t *srt = block_sort_keys(skeys, M);
+  for (uint32_t i = tid; i < a.S; i += 256) pq[i] = srt[i];
+  if (tid == 0) {
+    a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride] = a.S;
+    a.pool_sorted[q] = a.S;
+    const uint64_t kth = srt[a.S - 1];
+    if (kth < a.thr[q]) {
+      a.thr[q] = kth;
+      a.lbound[q] = static_cast<int32_t>(l2_bound(key_float(kth), a.step_sq, a.lmax));
+    }
+  }
+}
+
 template <int SC>
 __global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
   __shared__ uint64_t part[3][SC];
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x;
-  const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride),
-                 sorted = a.pool_sorted[q];
-  if (n == sorted) return;
+  const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride);
+  uint32_t sorted = a.pool_sorted[q];
+  if (sorted & 0x80000000u) {  // the selection kernel found too many candidates
+    sorted &= 0x7fffffffu;
+  } else if (n == sorted || update_by_selection(a.S, n)) {
+    return;
+  }
   uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
   TopList<SC> t;
 #pragma unroll
@@ -1369,6 +1445,8 @@ bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, ui
                    uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s) {
   UpdateArgs a{pool, pool_cnt, pool_sorted, thr, lbound, B, S, stride, static_cast<int64_t>(P) * 65025, step_sq};
   const dim3 grid(B);
+  // large pools by selection, then the rest (and any the selection left) by merging
+  if (S <= 256) hipLaunchKernelGGL(k_update_sel, grid, dim3(256), 0, s, a);
   switch (refine_cap(S)) {
     case 64: hipLaunchKernelGGL(k_update<64>, grid, dim3(256), 0, s, a); return true;
     case 128: hipLaunchKernelGGL(k_update<128>, grid, dim3(256), 0, s, a); return true;
