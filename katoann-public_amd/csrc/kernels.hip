@@ -716,6 +716,9 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
 //                     and its items; per 1024 slots: their sums,
 //   k_slot_scan       slot and item offsets (exclusive scans),
 //   k_inv_pass<true>  the query lists of every slot (ascending queries).
+// When nprobe is well below nlist (inv_from_probes) the two passes read the
+// probe lists instead: k_inv_probe, one thread per probed pair, its band from
+// its probe rank, counts and query-list cursors by atomics.
 __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
@@ -863,6 +866,56 @@ __global__ __launch_bounds__(256) void k_inv_pass(const float *__restrict__ dist
   if (!FILL)
 #pragma unroll
     for (int b = 0; b < NB; ++b) part[static_cast<size_t>(qt) * nslots + static_cast<size_t>(b) * nlist + c] = acc[b];
+}
+
+// Inversion from the probe lists (nprobe well below nlist: B x nprobe pairs
+// instead of the B x nlist matrix). Slot band of probe rank r (r0: the query's seed ranks when seed > 0).
+__device__ inline uint32_t rank_band(uint32_t r, uint32_t shift, uint32_t seed, uint32_t r0) {
+  const uint64_t F = 1ull << shift, cap = 1ull << 40;
+  if (seed) {
+    if (r < r0) return 0;
+    const uint64_t rr = r - r0;
+    uint64_t s = 0, size = F;
+    uint32_t b = 2;
+    while (rr >= s + size) {
+      s += size;
+      size = min(size * F, cap);
+      ++b;
+    }
+    return b;
+  }
+  uint64_t s = 0, size = 1;
+  uint32_t b = 0;
+  while (r >= s + size) {
+    s += size;
+    size = min(size * F, cap);
+    ++b;
+  }
+  return b;
+}
+
+// One thread per (query, probe rank) pair of the batch.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ probes, const uint32_t *__restrict__ npq,
+                                                   uint32_t B, uint32_t np, uint32_t nlist, uint32_t shift,
+                                                   const uint32_t *__restrict__ seed_r0, uint32_t seed,
+                                                   const uint64_t *__restrict__ list_off, uint32_t *__restrict__ part,
+                                                   const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= static_cast<uint64_t>(B) * np) return;
+  const uint32_t q = static_cast<uint32_t>(e / np), r = static_cast<uint32_t>(e % np);
+  const uint32_t n = npq ? min(npq[q], np) : np;
+  if (r >= n) return;
+  const uint32_t c = probes[e];
+  const uint32_t b = rank_band(r, shift, seed, seed ? seed_r0[q] : 0u);
+  const size_t slot = static_cast<size_t>(b) * nlist + c;
+  if (FILL) qlist[qoff[slot] + atomicAdd(&part[slot], 1u)] = q;
+  else atomicAdd(&part[slot], 1u);
+  if (seed && b == 0 && list_off[c + 1] - list_off[c] > seed) {
+    const size_t rest = static_cast<size_t>(nlist) + c;
+    if (FILL) qlist[qoff[rest] + atomicAdd(&part[rest], 1u)] = q;
+    else atomicAdd(&part[rest], 1u);
+  }
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -1108,7 +1161,7 @@ uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) 
 size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) {
   const size_t nslots = plan_slots(nlist, np, shift, seed);
   const size_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
-  return qtiles * nslots + nslots + 2 * nblk + B;
+  return (inv_from_probes(nlist, np) ? 1 : qtiles) * nslots + nslots + 2 * nblk + B;
 }
 
 template <bool FILL>
@@ -1136,7 +1189,8 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                  uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
                  uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
   const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
-  const uint32_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
+  const bool from_probes = inv_from_probes(nlist, np);
+  const uint32_t qtiles = from_probes ? 1u : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
   uint32_t *seed_r0 = bsum + 2 * nblk;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
@@ -1144,6 +1198,21 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   if (seed)
     hipLaunchKernelGGL(k_seed_ranks, dim3((B + 255) / 256), dim3(256), 0, s, probes, npq, list_off, B, np, seed,
                        seed_r0);
+  if (from_probes) {
+    const uint64_t pairs = static_cast<uint64_t>(B) * np;
+    const dim3 pgrid(static_cast<uint32_t>((pairs + 255) / 256));
+    hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, part, nslots, 0u);
+    hipLaunchKernelGGL(k_inv_probe<false>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
+                       list_off, part, nullptr, nullptr);
+    // (one partial count per slot: the counts become the slots' cursors, at 0)
+    hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, seed,
+                       gcnt, nit, bsum);
+    hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
+    hipLaunchKernelGGL(k_inv_probe<true>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
+                       list_off, part, qoff, qlist);
+    hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr, lbound);
+    return;
+  }
   inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                          nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, seed,

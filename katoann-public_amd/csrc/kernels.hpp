@@ -49,6 +49,11 @@ inline uint32_t plan_bands(uint32_t np, uint32_t shift, uint32_t seed = 0) {
   return nb;
 }
 uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed = 0);
+// The inversion reads the probe lists (B x nprobe pairs, atomic slot counters)
+// when nprobe is well below nlist, else the coarse matrix in query tiles
+// (ascending queries per slot, no atomics: lists probed by most of the batch
+// would serialise on their counters).
+inline bool inv_from_probes(uint32_t nlist, uint32_t np) { return 16ull * np <= nlist; }
 // 32-bit words of plan scratch for a batch (tile partial counts, item counts,
 // block sums, seed ranks).
 size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed = 0);
