@@ -272,100 +272,99 @@ __device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// Block tile: 128 queries x 128 centroids, 4 waves, wave (wq, wc) owns the
-// 64 x 64 quadrant = 4 x 4 MFMA tiles with three accumulators each (hh, hl + lh,
-// ll). Per 64-byte chunk of the codes the four operand planes (queries and
+// Per 64-byte chunk of the codes the four operand planes (queries and
 // centroids, high and low) are staged in LDS in MFMA fragment order (lane l's
 // 16 bytes at l * 16 of a 1 KB tile fragment, so every read is conflict-free);
 // the next chunk is loaded into registers while the current one is multiplied.
 constexpr int CO_T = 128;                    // queries and centroids per block tile
 constexpr int CO_STAGE = 4 * CO_T * 64;      // bytes of one chunk: 4 planes x 128 rows x 64
 
-__global__ __launch_bounds__(256, 1) void k_coarse(const int8_t *__restrict__ qhi, const int8_t *__restrict__ qlo,
-                                                   const int64_t *__restrict__ qnorm, uint32_t B,
-                                                   const int8_t *__restrict__ chi, const int8_t *__restrict__ clo,
-                                                   const int64_t *__restrict__ cnorm, uint32_t nlist, uint32_t dc,
-                                                   double step_sq, float *__restrict__ dist) {
+// Block tile 128 queries x 128 centroids over 8 waves; wave (wq, wc) owns 64
+// queries x 32 centroids = 4 x 2 MFMA tiles with three accumulators each (hh,
+// hl + lh, ll), which leaves room for two waves per SIMD (one hides the
+// other's LDS reads and barrier waits). Output norms are loaded ahead of the
+// main loop.
+__global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ qhi, const int8_t *__restrict__ qlo,
+                                                    const int64_t *__restrict__ qnorm, uint32_t B,
+                                                    const int8_t *__restrict__ chi, const int8_t *__restrict__ clo,
+                                                    const int64_t *__restrict__ cnorm, uint32_t nlist, uint32_t dc,
+                                                    double step_sq, float *__restrict__ dist) {
   __shared__ __attribute__((aligned(16))) int8_t st[2][CO_STAGE];
   const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-  const int wq = w >> 1, wc = w & 1;
+  const int wq = w >> 2, wc = w & 3;  // 64 queries (wq) x 32 centroids (wc)
   const uint32_t q0 = blockIdx.y * CO_T, c0 = blockIdx.x * CO_T;
-  // loader: thread t moves (row t / 4 and 64 + t / 4, slice t % 4) of each plane;
-  // plane order in a stage: query high, query low, centroid high, centroid low
+  // loader: thread t moves (row t / 4, slice t % 4) of each plane
   const int lr = tid >> 2, ls = tid & 3;
-  const int8_t *src[8];
-  uint32_t dst[8];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t r = static_cast<uint32_t>(lr + 64 * h);
+  const int8_t *src[4];
+  uint32_t dst[4];
+  {
+    const uint32_t r = static_cast<uint32_t>(lr);
     const uint32_t qr = min(q0 + r, B - 1), cr = min(c0 + r, nlist - 1);
-    src[0 + h] = qhi + static_cast<size_t>(qr) * dc + 16 * ls;
-    src[2 + h] = qlo + static_cast<size_t>(qr) * dc + 16 * ls;
-    src[4 + h] = chi + static_cast<size_t>(cr) * dc + 16 * ls;
-    src[6 + h] = clo + static_cast<size_t>(cr) * dc + 16 * ls;
-    // row r of a plane: tile r / 16, lane (slice, r % 16) of that tile's fragment
+    src[0] = qhi + static_cast<size_t>(qr) * dc + 16 * ls;
+    src[1] = qlo + static_cast<size_t>(qr) * dc + 16 * ls;
+    src[2] = chi + static_cast<size_t>(cr) * dc + 16 * ls;
+    src[3] = clo + static_cast<size_t>(cr) * dc + 16 * ls;
     const uint32_t off = (r >> 4) * 1024 + (ls * 16 + (r & 15)) * 16;
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) dst[2 * pl + h] = pl * (CO_T * 64) + off;
+    for (int pl = 0; pl < 4; ++pl) dst[pl] = pl * (CO_T * 64) + off;
   }
-  v4i pre[8];
+  v4i pre[4];
   auto fetch = [&](uint32_t kc) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) pre[i] = load16(src[i] + kc);
+    for (int i = 0; i < 4; ++i) pre[i] = load16(src[i] + kc);
   };
   auto stash = [&](int b) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) *reinterpret_cast<v4i *>(&st[b][dst[i]]) = pre[i];
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<v4i *>(&st[b][dst[i]]) = pre[i];
   };
-  v4i hh[4][4], mid[4][4], ll[4][4];
+  v4i hh[4][2], mid[4][2], ll[4][2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
+    for (int b = 0; b < 2; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
   const uint32_t nk = dc / 64;
   fetch(0);
-  // norms of the lane's output rows and columns, loaded behind the main loop
-  int64_t qn[4][4], cn[4];
+  int64_t qn[4][4], cn[2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int r = 0; r < 4; ++r) qn[a][r] = qnorm[min(q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r, B - 1)];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) cn[t] = cnorm[min(c0 + 64 * wc + 16 * t + (lane & 15), nlist - 1)];
+  for (int t = 0; t < 2; ++t) cn[t] = cnorm[min(c0 + 32 * wc + 16 * t + (lane & 15), nlist - 1)];
   stash(0);
   __syncthreads();
   for (uint32_t k = 0; k < nk; ++k) {
     const int b = k & 1;
     if (k + 1 < nk) fetch((k + 1) * 64);
     const int8_t *S = st[b];
-    // all of the step's fragments first: one wait for the LDS reads
-    v4i bh[4], bl[4], ah[4], al[4];
+    v4i bh[2], bl[2], ah[4], al[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      bh[t] = *reinterpret_cast<const v4i *>(S + 2 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
-      bl[t] = *reinterpret_cast<const v4i *>(S + 3 * CO_T * 64 + (4 * wc + t) * 1024 + lane * 16);
-      ah[t] = *reinterpret_cast<const v4i *>(S + (4 * wq + t) * 1024 + lane * 16);
-      al[t] = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + t) * 1024 + lane * 16);
+    for (int t = 0; t < 2; ++t) {
+      bh[t] = *reinterpret_cast<const v4i *>(S + 2 * CO_T * 64 + (2 * wc + t) * 1024 + lane * 16);
+      bl[t] = *reinterpret_cast<const v4i *>(S + 3 * CO_T * 64 + (2 * wc + t) * 1024 + lane * 16);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
+      ah[a] = *reinterpret_cast<const v4i *>(S + (4 * wq + a) * 1024 + lane * 16);
+      al[a] = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + a) * 1024 + lane * 16);
+    }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
         hh[a][t] = mfma(ah[a], bh[t], hh[a][t]);
         mid[a][t] = mfma(ah[a], bl[t], mid[a][t]);
         mid[a][t] = mfma(al[a], bh[t], mid[a][t]);
         ll[a][t] = mfma(al[a], bl[t], ll[a][t]);
       }
-    }
     if (k + 1 < nk) {
       stash(b ^ 1);
       __syncthreads();
     }
   }
-  // C[i][j] of tile (a, t) sits in lane 16 (i / 4) + j, element i % 4
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const uint32_t c = c0 + 64 * wc + 16 * t + (lane & 15);
+  for (int t = 0; t < 2; ++t) {
+    const uint32_t c = c0 + 32 * wc + 16 * t + (lane & 15);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1113,7 +1112,7 @@ void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, u
                    const int8_t *clo, const int64_t *cnorm, uint32_t nlist, uint32_t dc, double step_sq,
                    float *dist, hipStream_t s) {
   dim3 grid((nlist + CO_T - 1) / CO_T, (B + CO_T - 1) / CO_T);
-  hipLaunchKernelGGL(k_coarse, grid, dim3(256), 0, s, qhi, qlo, qnorm, B, chi, clo, cnorm, nlist, dc, step_sq,
+  hipLaunchKernelGGL(k_coarse8, grid, dim3(512), 0, s, qhi, qlo, qnorm, B, chi, clo, cnorm, nlist, dc, step_sq,
                      dist);
 }
 
