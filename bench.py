@@ -12,8 +12,11 @@ through the C ABI from device memory, queries resident in HBM.
 
 A step = one search of one 4,096-query batch through the whole pipeline
 (rotation, quantisation, coarse MFMA, probe selection, inversion, ranking,
-refinement, top-k). value = queries/s at the smallest swept nprobe whose
-recall@10 reaches 0.99.
+refinement, top-k). The timed loop keeps --inflight batches (default 3) in
+flight, each host thread searching on its own context and stream, as a
+serving loop does; stage times and the ranking roofline come from a separate
+pass of sequential eager searches. value = queries/s at the best swept point
+whose recall@10 reaches 0.99.
 """
 from __future__ import annotations
 
@@ -66,6 +69,8 @@ def main():
     ap.add_argument("--band-shift", type=int, default=0, help="probe-rank bands grow by 2**shift (default: library's)")
     ap.add_argument("--rank-kernel", default="", help="kernel,variant,queries_per_item (default: library's)")
     ap.add_argument("--seed-rows", type=int, default=-1, help="rows of the seed band (0 = none; default: library's)")
+    ap.add_argument("--inflight", type=int, default=3,
+                    help="batches in flight in the timed loop (each on its own search context and stream)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1024)
     ap.add_argument("--no-learned", action="store_true")
@@ -149,13 +154,32 @@ def main():
     sweep = []
     grid = [(int(a), int(b)) for b in args.survivors.split(",") for a in args.nprobe.split(",")
             if int(a) <= args.nlist]
+    # with several batches in flight, each host thread searches on its own
+    # context (stream, buffers, captured graph) into its own outputs
+    from concurrent.futures import ThreadPoolExecutor
+    pool = ThreadPoolExecutor(args.inflight) if args.inflight > 1 else None
+    outs = [(torch.empty((nq, k), dtype=torch.int32, device=dev), torch.empty((nq, k), dtype=torch.float32, device=dev))
+            for _ in range(args.inflight)]
+
+    def steps_on(j, n, np_, S):
+        for _ in range(n):
+            ix.search_device(qptr, nq, outs[j][0].data_ptr(), outs[j][1].data_ptr(), k=k, nprobe=np_, survivors=S)
+
+    def timed(n, np_, S):
+        if pool is None:
+            for _ in range(n):
+                step(np_, S)  # graph replays, no stage events
+            return
+        shares = [n // args.inflight + (1 if j < n % args.inflight else 0) for j in range(args.inflight)]
+        list(pool.map(lambda j: steps_on(j, shares[j], np_, S), range(args.inflight)))
+
     for np_, S in grid:
         for _ in range(args.warmup):
             step(np_, S)
+        timed(args.warmup * args.inflight if pool else 0, np_, S)  # the other contexts' graphs
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step(np_, S)  # graph replays, no stage events
+        timed(args.steps, np_, S)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         # the same steps again, eagerly, with HIP events around every stage and
@@ -239,7 +263,8 @@ def main():
         "config": {"workload": ("%dx%d mixture of 16384 components, Zipf(1.1) weights, i^-1 spectrum" % (
             args.n, args.dim) if zipf else "%dx%d i^-%g spectrum" % (args.n, args.dim, args.alpha)) +
             ", nlist %d, prefix %d, k %d, batch %d" % (args.nlist, pd, k, nq),
-            "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"]},
+            "nprobe": head["nprobe"], "survivors": head["survivors"], "recall_at_10": head["recall"],
+            "batches_in_flight": args.inflight},
         "qps_at_0999": b999["qps"] if b999 else None,
         "at_0999": {"nprobe": b999["nprobe"], "survivors": b999["survivors"]} if b999 else None,
         "roofline": dict(roofline(head), traffic=profiled_traffic(head, args)),
@@ -331,17 +356,33 @@ def learned_leg(ix, base, queries, gt, b, sizes, args, plain, ids_d, dists_d):
             break
     if margin is None:
         raise RuntimeError("no margin reaches recall 0.99 on the validation queries")
-    for _ in range(args.warmup):
-        run(queries, margin)
-    torch.cuda.synchronize()
+    # timed like the plain sweep: graph replays with the same batches in flight,
+    # each host thread on its own context and outputs; stage times from a
+    # separate eager pass
+    mstruct = model.struct(margin)
+    outs = [(torch.empty_like(ids_d), torch.empty_like(dists_d), torch.empty_like(used))
+            for _ in range(args.inflight)]
+
+    def run_on(j, n):
+        for _ in range(n):
+            ix.search_device(queries.data_ptr(), nq, outs[j][0].data_ptr(), outs[j][1].data_ptr(), k=args.k,
+                             nprobe=nmax, survivors=S, model=mstruct, npq_out_ptr=outs[j][2].data_ptr())
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(args.inflight) as pool:
+        list(pool.map(lambda j: run_on(j, args.warmup), range(args.inflight)))
+        torch.cuda.synchronize()
+        shares = [args.steps // args.inflight + (1 if j < args.steps % args.inflight else 0)
+                  for j in range(args.inflight)]
+        t0 = time.perf_counter()
+        list(pool.map(lambda j: run_on(j, shares[j]), range(args.inflight)))
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
     ix.set_timing(True)
     stage_sum = np.zeros(len(__import__("katoann").STAGES))
-    t0 = time.perf_counter()
     for _ in range(args.steps):
         run(queries, margin)
         stage_sum += np.array(list(ix.stage_times().values()))
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
     ix.set_timing(False)
     rec = recall_at(ids_d.cpu().numpy().view(np.uint32), gt, 10)
     mean_np = float(used.float().mean().item())

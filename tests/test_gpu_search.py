@@ -325,3 +325,33 @@ def test_selection_many_lists_vs_oracle(many_lists, nl):
     for nprobe in (1, 77, 4096):
         print("many lists", nl, nprobe, flush=True)
         check_against_oracle(ix, ox, q[:64], 10, nprobe, 32)
+
+
+def test_concurrent_searches_match_sequential(mixture_index):
+    """Searches from several host threads at once (each on its own search
+    context, stream and captured graph, shapes mixed and repeated) return what
+    the same searches return one at a time."""
+    import threading
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    shapes = [(10, 16, 32), (10, 64, 128), (100, 8, 100), (10, 32, 64)]
+    ref = {s: ix.search_ex(q[:500], k=s[0], nprobe=s[1], survivors=s[2], normalize=False)[:2] for s in shapes}
+    errors = []
+
+    def run(t):
+        try:
+            for rep in range(3):
+                for s in shapes[t % len(shapes):] + shapes[:t % len(shapes)]:
+                    i, d, _ = ix.search_ex(q[:500], k=s[0], nprobe=s[1], survivors=s[2], normalize=False)
+                    if not (np.array_equal(i, ref[s][0]) and np.array_equal(d.view(np.uint32), ref[s][1].view(np.uint32))):
+                        errors.append((t, rep, s))
+        except Exception as e:  # surfaced below
+            errors.append((t, repr(e)))
+
+    print("concurrent searches: 3 threads", flush=True)
+    ts = [threading.Thread(target=run, args=(t,)) for t in range(3)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
