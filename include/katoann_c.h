@@ -127,7 +127,7 @@ typedef struct katoann_search_extras {
   int32_t device_io;                /* queries, ids, dists (and the outputs below) are device pointers */
   uint32_t *probes;                 /* nq x nprobe: probed lists, nearest first (UINT32_MAX past a query's count) */
   float *probe_dists;               /* nq x nprobe: their coarse distances */
-  uint64_t *survivors;              /* nq x max(survivors, k): ranking keys (float bits << 32 | id), ascending */
+  uint64_t *survivors;              /* nq x max(survivors, k) (k for full ranking): ranking keys (float bits << 32 | id), ascending */
   uint32_t *n_survivors;            /* nq */
   float *rotated;                   /* nq x dim: rotated (normalised) queries */
   uint16_t *codes;                  /* nq x Dc: 16-bit query codes */

@@ -116,7 +116,7 @@ struct TileConfig {
   uint32_t kernel, variant, qc, shift;
   uint32_t seed;  // rows of the seed band (0: geometric bands from rank 0)
 };
-using TileKey = std::array<uint32_t, 6>;  // P, nlist, batch rounded up to 2^k, nprobe, S, prefix
+using TileKey = std::array<uint32_t, 6>;  // P, nlist, batch rounded up to 2^k, nprobe, S, ranking
 
 struct katoann_index {
   uint32_t dim = 0, nlist = 0, npts = 0, dc = 0, head_dims = 0;
@@ -436,7 +436,7 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
   uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
-  bool prefix, wave, ring;
+  bool prefix, full, wave, ring;
   uint64_t max_rows, allowance;
   size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
       codes, npq, mx16, mh, pool;
@@ -452,7 +452,7 @@ TileKey tile_key(const katoann_index &ix, uint32_t nq, const katoann_search_para
   uint32_t b = 1;
   while (b < nq) b <<= 1;
   return TileKey{prefix ? ix.head_dims : ix.dc, ix.nlist, b, p.nprobe, std::max(p.survivors, p.k),
-                 prefix ? 1u : 0u};
+                 static_cast<uint32_t>(p.ranking)};
 }
 
 // The tile a search runs with: the forced one, else the one measured fastest
@@ -475,30 +475,33 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
     throw std::invalid_argument("need k >= 1 and 1 <= nprobe <= nlist");
   if (p.ranking == KATOANN_RANK_PREFIX && ix.head_dims == 0)
     throw std::invalid_argument("prefix ranking needs the rotation file (.dimc)");
-  if (p.ranking == KATOANN_RANK_FULL)
-    throw std::invalid_argument("Ranking::kFull is not supported by the gfx950 engine yet");
-  if (p.ranking != KATOANN_RANK_PREFIX && p.ranking != KATOANN_RANK_HIGH)
+  if (p.ranking != KATOANN_RANK_PREFIX && p.ranking != KATOANN_RANK_HIGH && p.ranking != KATOANN_RANK_FULL)
     throw std::invalid_argument("ranking is 'prefix', 'high' or 'full'");
   if (p.selection != KATOANN_SEL_THRESHOLD && p.selection != KATOANN_SEL_LOCAL)
     throw std::invalid_argument("selection is 'threshold' or 'local'");
   SearchPlan z{};
-  z.S = std::max(p.survivors, p.k);
+  z.full = p.ranking == KATOANN_RANK_FULL;
+  // full ranking keeps the k nearest of all candidates (src/index.cpp:1130-1219)
+  z.S = z.full ? p.k : std::max(p.survivors, p.k);
   if (z.S > kMaxSurvivors)
     throw std::invalid_argument("max(survivors, k) above 1024 is not supported by the gfx950 engine");
   if (p.nprobe > kMaxProbes) throw std::invalid_argument("nprobe above 4096 is not supported by the gfx950 engine");
   z.prefix = p.ranking == KATOANN_RANK_PREFIX;
   z.P = z.prefix ? ix.head_dims : ix.dc;
-  if (z.P / 64 > 16) throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
+  if (!z.full && z.P / 64 > 16)
+    throw std::invalid_argument("high-byte ranking above 1024 dimensions is not supported yet");
   z.B = nq, z.d = ix.dim, z.dc = ix.dc, z.np = p.nprobe, z.K = p.k, z.nlist = ix.nlist;
   const TileConfig tc = tile_for(ix, nq, p, force);
   z.shift = tc.shift;
   // a seed band below S rows could leave a query without a threshold
-  z.seed = tc.seed ? std::max(tc.seed, std::max(p.survivors, p.k)) : 0u;
-  z.wave = tc.kernel >= 1;
-  z.ring = tc.kernel == 2;
-  z.variant = tc.variant;
-  z.qc = z.wave ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
-  z.rt = z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1) : ix.rank_rows;
+  z.seed = tc.seed ? std::max(tc.seed, z.S) : 0u;
+  z.wave = !z.full && tc.kernel >= 1;
+  z.ring = !z.full && tc.kernel == 2;
+  z.variant = z.full ? 0u : tc.variant;
+  z.qc = z.wave || z.full ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
+  z.rt = z.full   ? katoann_hip::rank_full_rows()
+         : z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1)
+                  : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
   z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift, z.seed);
   z.nbands = katoann_hip::plan_bands(z.np, z.shift, z.seed);
@@ -627,8 +630,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
   };
   // ranking blocks: as many as are resident at once
+  const bool full = z.full;
   const uint32_t grid = static_cast<uint32_t>(device_cus()) *
-                        (z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant) : 2u);
+                        (full                ? katoann_hip::rank_full_blocks_per_cu()
+                         : z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant)
+                                             : 2u);
   uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
   float *ddst = dev_io ? out_dists : c.dists.as<float>();
   const float *yq = c.q.as<float>();
@@ -695,8 +701,15 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const uint32_t qstride = prefix ? P : dc;
     for (uint32_t band = 0; band < nbands; ++band) {
       if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
+      if (full)
+        katoann_hip::launch_rank_full(band, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
+                                      ix.ids.as<uint32_t>(), nlist, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), dc,
+                                      c.qnorm.as<int64_t>(), c.thr.as<uint64_t>(), c.qlist.as<uint32_t>(),
+                                      c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride, ix.step_sq,
+                                      c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
       const bool launched =
-          z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
+          full ? true
+          : z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
                                                  qrank, qstride, qn, c.lbound.as<int32_t>(),
                                                  c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
                                                  c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
@@ -731,7 +744,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     if (graphs) {
       // everything a launch reads: shapes, scalars, every buffer, the stream
       std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
-                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, grid, ix.gen,
+                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, full, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
                                    x && x->survivors, x && x->n_survivors, x && x->codes};
@@ -780,7 +793,8 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   // on a pool overflow the batch runs again with 4x the room, up to every
   // candidate; later searches of the shape start from the room that sufficed
   const std::array<uint64_t, 8> akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
-                                        uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9)};
+                                        uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) |
+                                            (uint64_t(z.full) << 10)};
   uint64_t allowance = z.allowance;
   {
     std::lock_guard<std::mutex> g(mix.tune_mu);
@@ -961,7 +975,7 @@ namespace {
 void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
                 const katoann_search_extras *x, uint32_t *ids, float *dists) {
   auto &mix = const_cast<katoann_index &>(ix);
-  if (!mix.autotune.load() || nq == 0 || ix.nlist == 0) return;
+  if (!mix.autotune.load() || nq == 0 || ix.nlist == 0 || p.ranking == KATOANN_RANK_FULL) return;
   const TileKey key = tile_key(ix, nq, p);
   {
     std::lock_guard<std::mutex> g(mix.tune_mu);

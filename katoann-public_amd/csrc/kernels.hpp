@@ -92,6 +92,15 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
                       const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
                       uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
                       uint32_t grid, hipStream_t s);
+// Ranking over the full codes (exact distances, pools of the k smallest
+// keys): rows per item, and the launch of one band.
+uint32_t rank_full_rows();
+uint32_t rank_full_blocks_per_cu();
+void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const int64_t *xnorm, const uint32_t *ids,
+                      uint32_t nlist, const int8_t *qhi, const int8_t *qlo, uint32_t dc, const int64_t *qnorm,
+                      const uint64_t *thr, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                      uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                      uint32_t grid, hipStream_t s);
 uint32_t refine_cap(uint32_t S);
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
                    uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s);

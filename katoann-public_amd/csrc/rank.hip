@@ -945,6 +945,120 @@ __global__ __launch_bounds__(256, 2) void k_rank_ring(WaveArgs a) {
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
+// --------------------------------------------------------------------------
+// Ranking over the full 16-bit codes (Ranking::kFull, src/index.cpp:1130-1219):
+// the exact distance of every candidate, the four byte products on MFMA,
+//   dot' = 65536 (sh . sh) + 256 (sh . sl + sl . sh) + sl . sl,
+// in three int32 accumulators (each |sum| <= 2 dc 128^2 < 2^31), combined in
+// int64; key = (float bits of f(qn + xn - 2 dot')) << 32 | id. A key enters
+// its query's pool iff it is below the query's threshold key (the S-th
+// smallest of the pool so far, S = k), so each pool ends with the k smallest
+// (float, id) keys of all the query's candidates.
+// One wave per item (FULL_T x 16 rows, the item's queries 16 at a time), both
+// operands streamed per 64-byte k-slice: the item's rows through L2 (read
+// again for each query fragment), the queries from the batch's codes. Kept
+// keys reserve their pool slots with one atomic per 16-lane group (the lanes
+// of one query).
+struct FullArgs {
+  const int8_t *hi, *lo;     // N x dc, shifted bytes of the full codes
+  const int64_t *xnorm;      // N: shifted norms of the full codes
+  const uint32_t *ids;       // N
+  const int8_t *qhi, *qlo;   // B x dc
+  const int64_t *qnorm;      // B
+  const uint64_t *thr;       // B: threshold key (all ones while the pool holds fewer than S)
+  const uint32_t *qlist, *item_off;
+  const uint4 *items;
+  uint32_t nlist, band, dc, stride;
+  uint64_t *pool;
+  uint32_t *pool_cnt, *state;
+  double step_sq;
+};
+
+constexpr int FULL_T = 4;
+
+__global__ __launch_bounds__(256, 2) void k_rank_full(FullArgs a) {
+  constexpr int T = FULL_T;
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  const uint32_t nw = gridDim.x * 4;
+  const size_t dc = a.dc;
+  uint32_t n_kept = 0;  // wave-uniform
+  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw) {
+    const uint4 d = a.items[it];
+    const int8_t *xh[T], *xl[T];
+    int64_t xn[T];
+    uint32_t xid[T];
+    bool rv[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const uint32_t row = d.z + 16 * t + c16;
+      rv[t] = row < d.w;
+      const size_t rr = min(row, d.w - 1);
+      xh[t] = a.hi + rr * dc + 16 * g16;
+      xl[t] = a.lo + rr * dc + 16 * g16;
+      xn[t] = a.xnorm[rr];
+      xid[t] = a.ids[rr];
+    }
+    for (uint32_t f = 0; 16 * f < d.y; ++f) {
+      // A operand: query 16 f + (lane % 16), bytes 16 (lane / 16) of each k-slice
+      const uint32_t qa = a.qlist[d.x + min(16 * f + c16, d.y - 1)];
+      const int8_t *ph = a.qhi + qa * dc + 16 * g16, *pl = a.qlo + qa * dc + 16 * g16;
+      v4i hh[T], cr[T], ll[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) hh[t] = cr[t] = ll[t] = v4i{0, 0, 0, 0};
+#pragma unroll 2
+      for (uint32_t k = 0; k < dc; k += 64) {
+        const v4i qh = load16(ph + k), ql = load16(pl + k);
+        v4i bh[T], bl[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          bh[t] = load16(xh[t] + k);
+          bl[t] = load16(xl[t] + k);
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          hh[t] = mfma(qh, bh[t], hh[t]);
+          cr[t] = mfma(qh, bl[t], cr[t]);
+          cr[t] = mfma(ql, bh[t], cr[t]);
+          ll[t] = mfma(ql, bl[t], ll[t]);
+        }
+      }
+      // C[i][j] (query 16 f + i, row 16 t + j) sits in lane 16 (i / 4) + j,
+      // element i % 4: the 16 lanes of a group share one query
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t qi = 16 * f + 4 * g16 + r;
+        const uint32_t q = static_cast<uint32_t>(__shfl(static_cast<int>(qa), 4 * g16 + r));
+        const bool qok = qi < d.y;
+        const int64_t qn = a.qnorm[q];
+        const uint64_t th = a.thr[q];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const int64_t dot = static_cast<int64_t>(hh[t][r]) * 65536 + static_cast<int64_t>(cr[t][r]) * 256 +
+                              static_cast<int64_t>(ll[t][r]);
+          const uint64_t key = make_key(fscale(qn + xn[t] - 2 * dot, a.step_sq), xid[t]);
+          const bool keep = qok && rv[t] && key < th;
+          const uint64_t m = ballot(keep);
+          if (!m) continue;
+          const uint32_t gm = static_cast<uint32_t>(m >> (16 * g16)) & 0xffffu;
+          uint32_t base = 0;
+          if (c16 == 0 && gm)
+            base = atomicAdd(&a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride],
+                             static_cast<uint32_t>(__popc(gm)));
+          base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 16 * g16));
+          if (keep) {
+            const uint32_t pos = base + static_cast<uint32_t>(__popc(gm & ((1u << c16) - 1u)));
+            if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+            else atomicOr(&a.state[0], 1u);
+          }
+          n_kept += static_cast<uint32_t>(__popcll(m));
+        }
+      }
+    }
+  }
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
 // After a band: each query's pool back to its S smallest keys, ascending, and
 // the query's threshold to the S-th of them (an upper bound of its final S-th
 // key), with the integer l2 bound of that threshold for the next band's
@@ -1439,6 +1553,19 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
     default:
       return false;
   }
+}
+
+uint32_t rank_full_rows() { return 16 * FULL_T; }
+uint32_t rank_full_blocks_per_cu() { return 3; }  // 142 VGPRs: three waves per SIMD
+
+void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const int64_t *xnorm, const uint32_t *ids,
+                      uint32_t nlist, const int8_t *qhi, const int8_t *qlo, uint32_t dc, const int64_t *qnorm,
+                      const uint64_t *thr, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                      uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                      uint32_t grid, hipStream_t s) {
+  const FullArgs a{hi, lo, xnorm, ids, qhi, qlo, qnorm, thr, qlist, item_off, items, nlist, band, dc, stride,
+                   pool, pool_cnt, state, step_sq};
+  hipLaunchKernelGGL(k_rank_full, dim3(grid), dim3(256), 0, s, a);
 }
 
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,

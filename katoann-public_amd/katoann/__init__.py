@@ -274,7 +274,7 @@ class Index:
         x = _Extras()
         x.normalize = int(self.normalized if normalize is None else normalize)
         extra = {}
-        S = max(survivors, k)
+        S = k if ranking == "full" else max(survivors, k)
         npq = None
         if nprobe_per_query is not None:
             npq = np.ascontiguousarray(nprobe_per_query, dtype=np.uint32)

@@ -33,6 +33,15 @@ def compare(ids, dists, ref_ids, ref_dists, tie_rows):
     return int(ok.sum()), int(tie.sum())
 
 
+def tie_mask(ranking, selection):
+    """Oracle flag bits that mark a query whose reference result may differ
+    from the canonical one: a coarse tie (1), a survivor tie under threshold
+    selection (2), a tie at the k-th exact distance of full ranking (4)."""
+    if ranking == "full":
+        return 5
+    return 1 if selection == "local" else 3
+
+
 def golden_names():
     return sorted(n for n in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, n)))
 

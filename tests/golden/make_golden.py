@@ -43,6 +43,10 @@ def grid(nlist):
             for nprobe in (1, 4, min(16, nlist)):
                 for k, S in ((10, 10), (10, 32), (10, 128), (32, 32)):
                     out.append((k, nprobe, S, ranking, selection))
+    # Ranking::kFull keeps the k nearest of all candidates (survivors unused)
+    for nprobe in (1, 4, min(16, nlist)):
+        for k in (10, 32):
+            out.append((k, nprobe, k, "full", "threshold"))
     return out
 
 

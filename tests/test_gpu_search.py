@@ -48,7 +48,7 @@ def test_search_matches_reference(name):
         ids, dists, _ = ix.search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, selection=selection,
                                      normalize=False)
         flags = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
-        tie = (flags & (1 if selection == "local" else 3)) != 0
+        tie = (flags & common.tie_mask(ranking, selection)) != 0
         n_ok, _ = common.compare(ids, dists, ref["ids_" + tag], ref["dists_" + tag], tie)
         rows += n_ok
     assert rows > 0

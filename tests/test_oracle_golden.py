@@ -44,7 +44,7 @@ def test_search_matches_reference(name):
             assert np.array_equal(o["dists"].view(np.uint32), r_d.view(np.uint32)), tag
         # canonical ties: bit-exact away from tie rows, the reference's same() on them
         c = ix.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)
-        tie = (c["flags"] & (1 if selection == "local" else 3)) != 0
+        tie = (c["flags"] & common.tie_mask(ranking, selection)) != 0
         n_ok, _ = common.compare(c["ids"], c["dists"], r_ids, r_d, tie)
         exact_rows += n_ok
     assert exact_rows > 0
