@@ -143,6 +143,7 @@ struct katoann_index {
   uint32_t tile_prefix_rows = 0;
 
   DevBuf rot, list_off_d, ids, id_row, hi, lo_plane, pre, norm, hinorm, headnorm, cen_hi, cen_lo, cen_norm;
+  DevBuf cen_f;  // float centroids, unrotated: the per-query coarse search (batched = false)
 
   std::mutex mu;
   std::vector<std::unique_ptr<SearchCtx>> free_ctx;
@@ -171,7 +172,7 @@ struct katoann_index {
     tile_prefix.clear();
     tile_prefix_rows = 0;
     for (DevBuf *b : {&rot, &list_off_d, &ids, &id_row, &hi, &lo_plane, &pre, &norm, &hinorm, &headnorm, &cen_hi,
-                      &cen_lo, &cen_norm})
+                      &cen_lo, &cen_norm, &cen_f})
       b->release();
   }
 
@@ -306,8 +307,6 @@ void check_load_options(const katoann_load_options &o) {
   if (o.precision == KATOANN_BF16)
     throw std::invalid_argument("bf16 precision is not supported by the gfx950 engine");
   if (o.precision != KATOANN_SQ16) throw std::invalid_argument("precision is 'sq16' or 'bf16'");
-  if (!o.batched_coarse)
-    throw std::invalid_argument("per-query flat coarse search is not supported by the gfx950 engine");
 }
 
 // Everything of Index::load after reading the files (src/index.cpp:777-807):
@@ -315,8 +314,9 @@ void check_load_options(const katoann_load_options &o) {
 // prefix plane, the centroid codes and the id map, all on the GPU.
 void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_t npts, bool normalized,
                         const float *centroids, const uint32_t *sizes, const uint32_t *ids, const float *vecs,
-                        bool vecs_on_device, const float *rotation, bool has_dimc, uint32_t dims) {
+                        bool vecs_on_device, const float *rotation, bool has_dimc, uint32_t dims, bool batched) {
   ix.reset();
+  ix.batched = batched;
   ix.dim = dim;
   ix.nlist = nlist;
   ix.npts = npts;
@@ -418,6 +418,9 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipDeviceSynchronize());
   }
+  // the per-query coarse search compares unrotated queries with the float
+  // centroids (src/index.cpp:832-844)
+  if (!batched) upload(ix.cen_f, centroids, static_cast<size_t>(nlist) * d);
   // ids and id -> row (src/index.cpp:795-806; a later duplicate wins)
   upload(ix.ids, ids, N);
   {
@@ -658,9 +661,12 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                  c.qh0.as<int32_t>(), prefix ? c.qpre.as<int8_t>() : nullptr,
                                  (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
     mark(1);
-    katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
-                               ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist,
-                               dc, ix.step_sq, c.dist.as<float>(), s);
+    if (ix.batched)
+      katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
+                                 ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist,
+                                 dc, ix.step_sq, c.dist.as<float>(), s);
+    else
+      katoann_hip::launch_coarse_flat(c.q.as<float>(), B, d, ix.cen_f.as<float>(), nlist, c.dist.as<float>(), s);
     mark(2);
     katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
                                s);
@@ -941,7 +947,7 @@ int katoann_load(katoann_index *h, const char *prefix_c, const katoann_load_opti
   const bool has_dimc = read_rotation(prefix, r, &stored, &rotated, rotation);
   const uint32_t dims = o.prefix_dims ? o.prefix_dims : stored;
   build_device_index(*h, r.dim, r.nlist, r.npts, r.normalized, r.centroids.data(), r.sizes.data(), r.ids.data(),
-                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims);
+                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims, o.batched_coarse != 0);
   KATOANN_CATCH
 }
 
@@ -961,7 +967,7 @@ int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t
   if (total != n) throw std::runtime_error("list sizes do not add up to n");
   const uint32_t dims = o.prefix_dims ? o.prefix_dims : prefix_dims;
   build_device_index(*h, dim, nlist, n, normalized != 0, centroids, list_sizes, ids, vectors,
-                     vectors_on_device != 0, rotation, true, dims);
+                     vectors_on_device != 0, rotation, true, dims, o.batched_coarse != 0);
   KATOANN_CATCH
 }
 
@@ -1071,7 +1077,7 @@ int32_t katoann_engine(const katoann_index *) { return KATOANN_ENGINE_GFX950; }
 int32_t katoann_precision(const katoann_index *h) { return h->precision; }
 uint32_t katoann_prefix_dims(const katoann_index *h) { return h->head_dims; }
 int32_t katoann_rotated(const katoann_index *h) { return h->rotated ? 1 : 0; }
-int32_t katoann_batched_coarse(const katoann_index *h) { return h->nlist ? 1 : 0; }
+int32_t katoann_batched_coarse(const katoann_index *h) { return h->nlist && h->batched ? 1 : 0; }
 
 int katoann_list_sizes(const katoann_index *h, uint32_t *out) {
   KATOANN_TRY
@@ -1082,7 +1088,7 @@ int katoann_list_sizes(const katoann_index *h, uint32_t *out) {
 uint64_t katoann_memory_bytes(const katoann_index *h) {
   uint64_t s = 0;
   for (const DevBuf *b : {&h->rot, &h->list_off_d, &h->ids, &h->id_row, &h->hi, &h->lo_plane, &h->pre, &h->norm,
-                          &h->hinorm, &h->headnorm, &h->cen_hi, &h->cen_lo, &h->cen_norm})
+                          &h->hinorm, &h->headnorm, &h->cen_hi, &h->cen_lo, &h->cen_norm, &h->cen_f})
     s += b->bytes;
   return s;
 }

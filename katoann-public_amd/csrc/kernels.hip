@@ -37,6 +37,13 @@ __global__ void k_normalize(float *x, uint32_t n, uint32_t d) {
 // addition is commutative, so every lane of the butterfly ends with the same
 // bits); then FMAs over k >= d & ~15 in order.
 //
+// The same kernel with L2 = true computes the per-query coarse distances
+// l2_fp32(x_i, c_j) (src/index.cpp:36-56, 832-844) of queries and float
+// centroids: chains of fmaf(x - c, x - c, acc), the same tree, and a tail
+// k >= d & ~15 summed separately from 0 (as the reference's AVX-512 build
+// evaluates it: when it has 8 or more terms, the first 8 are rounded products
+// added in k order, the rest FMAs), then tree + tail.
+//
 // Tile: 64 rows x 32 basis rows per 512-thread block. Thread = (chain l,
 // slot); the 4 slots of a wave share 8 input rows (LDS broadcast) and own 8
 // basis rows each, so each thread keeps 8 x 8 chain accumulators. The next
@@ -47,8 +54,10 @@ constexpr int RT_COLS = 32;
 constexpr int RT_K = 64;
 constexpr int RT_LD = RT_K + 2;  // 66 floats: the four slots of a wave hit disjoint banks
 
+template <bool L2>
 __global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uint32_t n, uint32_t d,
-                                                const float *__restrict__ R, float *__restrict__ Y) {
+                                                const float *__restrict__ R, uint32_t ncols,
+                                                float *__restrict__ Y, uint32_t ldy) {
   __shared__ float xs[2][RT_ROWS][RT_LD];
   __shared__ float rs[2][RT_COLS][RT_LD];
   const int tid = threadIdx.x;
@@ -56,13 +65,13 @@ __global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uin
   const int slot = tid >> 4;
   const int rg = slot >> 2, cg = slot & 3;
   const uint32_t i0 = blockIdx.x * RT_ROWS, j0 = blockIdx.y * RT_COLS;  // rows on x: no 65535 limit
-  const uint32_t d16 = d & ~15u, d4 = d & ~3u;
+  const uint32_t d16 = d & ~15u;
   // loader: thread t moves 4 consecutive k of input rows t / 16 and 32 + t / 16
   // and of basis row t / 16 per chunk
   const int lrow = tid >> 4, lk = 4 * (tid & 15);
   const float *xrow0 = X + static_cast<size_t>(min(i0 + lrow, n - 1)) * d;
   const float *xrow1 = X + static_cast<size_t>(min(i0 + 32 + lrow, n - 1)) * d;
-  const float *rrow = R + static_cast<size_t>(min(j0 + lrow, d - 1)) * d;
+  const float *rrow = R + static_cast<size_t>(min(j0 + lrow, ncols - 1)) * d;
   const bool vec = (d & 3u) == 0;
   float4 p0, p1, pr;
   auto ld4 = [&](const float *r, uint32_t k) {
@@ -116,7 +125,14 @@ __global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uin
 #pragma unroll
         for (int a = 0; a < 8; ++a)
 #pragma unroll
-          for (int b = 0; b < 8; ++b) acc[a][b] = fmaf(xv[a], rv[b], acc[a][b]);
+          for (int b = 0; b < 8; ++b) {
+            if (L2) {
+              const float dd = xv[a] - rv[b];
+              acc[a][b] = fmaf(dd, dd, acc[a][b]);
+            } else {
+              acc[a][b] = fmaf(xv[a], rv[b], acc[a][b]);
+            }
+          }
       }
     }
     if (more) {
@@ -148,11 +164,28 @@ __global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uin
       for (int bb = 0; bb < 8; ++bb)
         if (aa == a && bb == b) s = acc[aa][bb];
     const uint32_t i = i0 + rg * 8 + a, j = j0 + cg * 8 + b;
-    if (i < n && j < d4) {
+    if (i < n && j < ncols) {
       const float *x = X + static_cast<size_t>(i) * d;
       const float *r = R + static_cast<size_t>(j) * d;
-      for (uint32_t k = d16; k < d; ++k) s = fmaf(x[k], r[k], s);
-      Y[static_cast<size_t>(i) * d + j] = s;
+      if (L2) {
+        float t = 0.0f;
+        uint32_t k = d16;
+        if (d - d16 >= 8) {
+          for (; k < d16 + 8; ++k) {
+            const float dd = x[k] - r[k];
+            const float p = dd * dd;
+            t = t + p;
+          }
+        }
+        for (; k < d; ++k) {
+          const float dd = x[k] - r[k];
+          t = fmaf(dd, dd, t);
+        }
+        s = s + t;
+      } else {
+        for (uint32_t k = d16; k < d; ++k) s = fmaf(x[k], r[k], s);
+      }
+      Y[static_cast<size_t>(i) * ldy + j] = s;
     }
   }
 }
@@ -1087,13 +1120,20 @@ void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float
   const uint32_t d4 = d & ~3u;
   if (d4) {
     dim3 grid((n + RT_ROWS - 1) / RT_ROWS, (d4 + RT_COLS - 1) / RT_COLS);
-    hipLaunchKernelGGL(k_rotate, grid, dim3(512), 0, s, X, n, d, R, Y);
+    hipLaunchKernelGGL(k_rotate<false>, grid, dim3(512), 0, s, X, n, d, R, d4, Y, d);
   }
   if (d4 < d) {
     const uint64_t t = static_cast<uint64_t>(n) * (d - d4);
     hipLaunchKernelGGL(k_rotate_tail, dim3(static_cast<uint32_t>((t + 255) / 256)), dim3(256), 0, s, X, n, d, R,
                        Y);
   }
+}
+
+void launch_coarse_flat(const float *Q, uint32_t B, uint32_t d, const float *cen, uint32_t nlist, float *dist,
+                        hipStream_t s) {
+  if (!B || !nlist) return;
+  dim3 grid((B + RT_ROWS - 1) / RT_ROWS, (nlist + RT_COLS - 1) / RT_COLS);
+  hipLaunchKernelGGL(k_rotate<true>, grid, dim3(512), 0, s, Q, B, d, cen, nlist, dist, nlist);
 }
 
 void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32_t P, float lo,

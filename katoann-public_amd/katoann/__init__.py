@@ -248,6 +248,11 @@ class Index:
     def rotated(self):
         return bool(lib().katoann_rotated(self._h))
 
+    @property
+    def batched_coarse(self):
+        """False when loaded with batched_coarse=False (per-query fp32 coarse search)."""
+        return bool(lib().katoann_batched_coarse(self._h))
+
     def list_sizes(self):
         out = np.zeros(self.nlist, np.uint32)
         _check(lib().katoann_list_sizes(self._h, _ptr(out)))

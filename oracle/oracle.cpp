@@ -10,6 +10,8 @@
 // Pinning (how each part is checked, see tests/test_oracle_*.py):
 //   rotate_rows      golden vectors from the reference build (oracle/_ref) and
 //                    an fp64 bound; order pinned bit-exact by the golden files.
+//   l2_flat          the reference build's per-query coarse order of every
+//                    list (golden), an fp64 bound.
 //   quantize/norms   closed form, brute force, golden scale (lo, step_sq).
 //   sq16 distances   the reference's own kernel test formula
 //                    (tests/test_kernels.cpp:66-108), plus golden outputs.
@@ -59,6 +61,8 @@ struct OIndex {
   std::vector<uint16_t> ccodes;    // nlist x dc
   std::vector<int64_t> cnorm;      // nlist
   std::vector<uint32_t> id_row;    // npts -> row or UINT32_MAX
+  std::vector<float> cen;          // nlist x d float centroids, unrotated (per-query coarse search)
+  bool flat = false;               // LoadOptions::batched_coarse = false
 };
 
 // Reference: rotate_rows / rotate_tile4x4 / rotate_rows_serial
@@ -97,6 +101,43 @@ float rotate_one(const float *x, const float *r, uint32_t d, uint32_t j) {
   float s = t80 + t81;
   for (; k < d; ++k) s = std::fmaf(x[k], r[k], s);
   return s;
+}
+
+// Reference: l2_fp32 (index.cpp:36-56), the per-query coarse distance
+// (index.cpp:832-844), as GCC 11 -O3 compiles it for AVX-512 (read from the
+// assembly of the reference built here, pinned against its coarse_flat_ order):
+// sixteen fmaf chains of (a - b)^2, chain l over k = l, l+16, ... (k < d & ~15),
+// summed by the same _mm512_reduce_add_ps tree as the rotation; the tail
+// k >= d & ~15 is a separate sum from 0: when it has 8 or more terms, its first
+// 8 are rounded products added in k order (vectorised, no contraction), the
+// rest FMAs; then tree + tail.
+float l2_flat(const float *a, const float *b, uint32_t d) {
+  float acc[16];
+  for (int l = 0; l < 16; ++l) acc[l] = 0.0f;
+  uint32_t k = 0;
+  for (; k + 16 <= d; k += 16)
+    for (int l = 0; l < 16; ++l) {
+      const float dd = a[k + l] - b[k + l];
+      acc[l] = std::fmaf(dd, dd, acc[l]);
+    }
+  float tail = 0.0f;
+  if (d - k >= 8) {
+    for (uint32_t e = k + 8; k < e; ++k) {
+      const float dd = a[k] - b[k];
+      const float p = dd * dd;
+      tail = tail + p;
+    }
+  }
+  for (; k < d; ++k) {
+    const float dd = a[k] - b[k];
+    tail = std::fmaf(dd, dd, tail);
+  }
+  float t3[8], t6[4];
+  for (int i = 0; i < 8; ++i) t3[i] = acc[i + 8] + acc[i];
+  for (int i = 0; i < 4; ++i) t6[i] = t3[i + 4] + t3[i];
+  const float t80 = t6[0] + t6[2], t81 = t6[1] + t6[3];
+  const float tree = t80 + t81;
+  return tree + tail;
 }
 
 void rotate_rows(const float *in, float *out, size_t n, uint32_t d, const float *rot) {
@@ -221,6 +262,7 @@ void *oracle_index_from_raw(uint32_t d, uint32_t nlist, uint32_t npts, const uin
   for (uint32_t c = 0; c < nlist; ++c) ix->list_off[c + 1] = ix->list_off[c] + sizes[c];
   const uint64_t N = ix->list_off[nlist];
   ix->ids.assign(ids, ids + N);
+  ix->cen.assign(centroids, centroids + static_cast<size_t>(nlist) * d);
   std::vector<float> rot(static_cast<size_t>(N) * d);
   std::vector<float> crot(static_cast<size_t>(nlist) * d);
   if (rotation) {
@@ -287,6 +329,27 @@ void *oracle_index_from_codes(uint32_t d, uint32_t nlist, uint32_t npts, const u
 }
 
 void oracle_index_free(void *h) { delete static_cast<OIndex *>(h); }
+
+// Coarse search per query in fp32 (LoadOptions::batched_coarse = false);
+// needs the float centroids (an index from raw vectors). Returns 0 on success.
+int oracle_index_set_flat(void *h, int flat) {
+  auto *ix = static_cast<OIndex *>(h);
+  if (flat && ix->cen.empty()) return 1;
+  ix->flat = flat != 0;
+  return 0;
+}
+
+// The per-query coarse distances of every (query, list), nq x nlist.
+int oracle_flat_dists(const void *h, const float *queries, uint32_t nq, float *out) {
+  const auto *ix = static_cast<const OIndex *>(h);
+  if (ix->cen.empty()) return 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < static_cast<int64_t>(nq); ++i)
+    for (uint32_t c = 0; c < ix->nlist; ++c)
+      out[static_cast<size_t>(i) * ix->nlist + c] =
+          l2_flat(queries + static_cast<size_t>(i) * ix->d, ix->cen.data() + static_cast<size_t>(c) * ix->d, ix->d);
+  return 0;
+}
 
 void oracle_index_scale(const void *h, float *lo, double *inv_step, double *step_sq) {
   const auto *ix = static_cast<const OIndex *>(h);
@@ -368,8 +431,14 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
       int64_t qn, qhn, qh0n;
       code_norms(q, d, dc, ix->P, &qn, &qhn, &qh0n);
       uint8_t flg = 0;
-      // coarse (index.cpp:846-877, kernels.cpp:166-168 with M = 16)
-      for (uint32_t c = 0; c < nlist; ++c) {
+      // coarse (index.cpp:846-877, kernels.cpp:166-168 with M = 16), or per
+      // query in fp32 (index.cpp:832-844): its max-heap with strict admission
+      // over lists in index order keeps exactly the (float, list) order, ties
+      // included
+      if (ix->flat)
+        for (uint32_t c = 0; c < nlist; ++c)
+          cd[c] = l2_flat(queries + static_cast<size_t>(qi) * d, ix->cen.data() + static_cast<size_t>(c) * d, d);
+      for (uint32_t c = 0; c < nlist && !ix->flat; ++c) {
         const uint16_t *x = ix->ccodes.data() + static_cast<size_t>(c) * dc;
         int64_t dot = 0;
         for (uint32_t t = 0; t < dc; ++t) dot += static_cast<int64_t>(q[t]) * x[t];
@@ -377,7 +446,7 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
       }
       const uint32_t npq = nprobe_per_query ? std::min(nprobe_per_query[qi], nprobe) : nprobe;
       std::iota(ord.begin(), ord.end(), 0u);
-      if (libstdc_ties) {
+      if (libstdc_ties && !ix->flat) {
         auto lt = [&](uint32_t a, uint32_t b) { return cd[a] < cd[b]; };
         if (npq < nlist) std::nth_element(ord.begin(), ord.begin() + npq, ord.end(), lt);
         std::sort(ord.begin(), ord.begin() + npq, lt);
@@ -396,7 +465,7 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
           nxt = std::numeric_limits<float>::max();
           for (uint32_t p = npq; p < nlist; ++p) nxt = std::min(nxt, cd[ord[p]]);
         }
-        if (kth == nxt) flg |= 1;
+        if (kth == nxt && !ix->flat) flg |= 1;
       }
       if (so.probes)
         for (uint32_t p = 0; p < nprobe; ++p)

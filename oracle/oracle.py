@@ -55,6 +55,10 @@ def lib():
         L.oracle_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
         L.oracle_sq16_high_l2.argtypes = [P, u32, P, u32, P, P, f64, u32, u32, u32, P]
         L.oracle_normalize_rows.argtypes = [P, u64, u32]
+        L.oracle_index_set_flat.restype = i32
+        L.oracle_index_set_flat.argtypes = [P, i32]
+        L.oracle_flat_dists.restype = i32
+        L.oracle_flat_dists.argtypes = [P, P, u32, P]
         _lib = L
     return _lib
 
@@ -140,6 +144,20 @@ class OracleIndex:
             lib().oracle_index_free(self.h)
             self.h = None
 
+    def set_flat(self, flat=True):
+        """Per-query fp32 coarse search (the reference's batched_coarse = false)."""
+        if lib().oracle_index_set_flat(self.h, 1 if flat else 0) != 0:
+            raise ValueError("the per-query coarse search needs the float centroids")
+        return self
+
+    def flat_dists(self, q):
+        """Per-query coarse distances of every list, nq x nlist (unrotated queries)."""
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.empty((q.shape[0], self.nlist), np.float32)
+        if lib().oracle_flat_dists(self.h, _p(q), q.shape[0], _p(out)) != 0:
+            raise ValueError("the per-query coarse search needs the float centroids")
+        return out
+
     def scale(self):
         lo, inv, ssq = C.c_float(), C.c_double(), C.c_double()
         lib().oracle_index_scale(self.h, C.byref(lo), C.byref(inv), C.byref(ssq))
@@ -220,12 +238,12 @@ def read_ref_dump(path: str):
 
 
 def ref_search(prefix, queries_fbin, k, nprobe, survivors, ranking="prefix", selection="local",
-               batch=4096, out=None):
+               batch=4096, out=None, coarse="batched"):
     """Runs the reference Index::search (AVX-512 engine) through the driver."""
     import tempfile
     tmp = out or tempfile.mktemp(suffix=".bin")
     subprocess.run([REF_SEARCH, "search", prefix, queries_fbin, tmp, str(k), str(nprobe),
-                    str(survivors), ranking, selection, str(batch)], check=True)
+                    str(survivors), ranking, selection, str(batch), "0", coarse], check=True)
     raw = np.fromfile(tmp, dtype=np.uint8)
     if out is None:
         os.remove(tmp)
@@ -233,3 +251,14 @@ def ref_search(prefix, queries_fbin, k, nprobe, survivors, ranking="prefix", sel
     ids = raw[:4 * nqk].view(np.uint32)
     dists = raw[4 * nqk:].view(np.float32)
     return ids, dists
+
+
+def ref_flat_order(prefix, queries_fbin, nq, nlist):
+    """The reference's per-query coarse order of every list (Index::coarse_flat_
+    with nprobe = nlist) through the driver, nq x nlist."""
+    import tempfile
+    tmp = tempfile.mktemp(suffix=".bin")
+    subprocess.run([REF_SEARCH, "flat", prefix, queries_fbin, tmp], check=True)
+    out = np.fromfile(tmp, dtype=np.uint32).reshape(nq, nlist)
+    os.remove(tmp)
+    return out

@@ -9,8 +9,12 @@
 //       rotation of every list (index.cpp:613-629)
 //   katoann_ref_search search PREFIX QUERIES.fbin OUT.bin k nprobe survivors
 //                             ranking(full|high|prefix) selection(threshold|local)
-//                             [batch=4096] [normalize=0]
-//       ids (u32) and dists (f32), nq x k, from Index::search in batches
+//                             [batch=4096] [normalize=0] [coarse=batched|flat]
+//       ids (u32) and dists (f32), nq x k, from Index::search in batches;
+//       coarse=flat loads with LoadOptions::batched_coarse = false
+//   katoann_ref_search flat   PREFIX QUERIES.fbin OUT.bin
+//       per query, every list in the order of the per-query coarse search
+//       (Index::coarse_flat_ with nprobe = nlist), u32, nq x nlist
 //
 // The loaded state is read through the class's private members; the access
 // specifiers are opened for this translation unit only (the object layout of
@@ -127,6 +131,7 @@ int cmd_search(int argc, char **argv) {
   Index idx;
   katoann::LoadOptions lo;
   lo.use_amx = false;
+  lo.batched_coarse = !(argc > 12 && std::string(argv[12]) == "flat");
   idx.load(prefix, lo);
   uint32_t nq = 0, d = 0;
   std::vector<float> q = read_fbin(qpath, nq, d);
@@ -148,6 +153,23 @@ int cmd_search(int argc, char **argv) {
   return f ? 0 : 1;
 }
 
+int cmd_flat(const std::string &prefix, const std::string &qpath, const std::string &out) {
+  Index idx;
+  katoann::LoadOptions lo;
+  lo.use_amx = false;
+  lo.batched_coarse = false;
+  idx.load(prefix, lo);
+  uint32_t nq = 0, d = 0;
+  std::vector<float> q = read_fbin(qpath, nq, d);
+  const uint32_t nlist = idx.nlist();
+  std::vector<uint32_t> order(static_cast<size_t>(nq) * nlist);
+  for (uint32_t i = 0; i < nq; ++i)
+    idx.coarse_flat_(q.data() + static_cast<size_t>(i) * d, nlist, order.data() + static_cast<size_t>(i) * nlist);
+  std::ofstream f(out, std::ios::binary);
+  put(f, order.data(), order.size() * 4);
+  return f ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -156,10 +178,11 @@ int main(int argc, char **argv) {
       return cmd_dump(argv[2], argv[3], argc > 4 ? static_cast<uint32_t>(std::stoul(argv[4])) : 0u);
     if (argc >= 4 && std::string(argv[1]) == "rotate") return cmd_rotate(argv[2], argv[3]);
     if (argc >= 2 && std::string(argv[1]) == "search") return cmd_search(argc, argv);
+    if (argc >= 5 && std::string(argv[1]) == "flat") return cmd_flat(argv[2], argv[3], argv[4]);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "katoann_ref_search: %s\n", e.what());
     return 1;
   }
-  std::fprintf(stderr, "usage: katoann_ref_search dump|search ...\n");
+  std::fprintf(stderr, "usage: katoann_ref_search dump|rotate|search|flat ...\n");
   return 2;
 }

@@ -35,18 +35,36 @@ def compare(ids, dists, ref_ids, ref_dists, tie_rows):
 
 def tie_mask(ranking, selection):
     """Oracle flag bits that mark a query whose reference result may differ
-    from the canonical one: a coarse tie (1), a survivor tie under threshold
-    selection (2), a tie at the k-th exact distance of full ranking (4)."""
+    from the canonical one: a coarse tie (1; never set for the per-query coarse
+    search, whose order is fixed), a survivor tie under threshold selection
+    (2), a tie at the k-th exact distance of full ranking (4)."""
     if ranking == "full":
         return 5
     return 1 if selection == "local" else 3
 
 
-def golden_names():
+def _golden_dirs():
     return sorted(n for n in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, n)))
 
 
+def _golden_keys(name):
+    return np.load(os.path.join(GOLDEN, name, "reference_outputs.npz")).files
+
+
+def golden_names():
+    """Fixtures with the reference's load state and search outputs."""
+    return [n for n in _golden_dirs() if "codes_sha" in _golden_keys(n)]
+
+
+def flat_golden_names():
+    """Fixtures with the reference's per-query coarse order of every list."""
+    return [n for n in _golden_dirs() if "flat_order" in _golden_keys(n)]
+
+
 def golden(name):
+    """(index prefix, queries, reference outputs, cases); a case is
+    (k, nprobe, survivors, ranking, selection, coarse), coarse "batched" or
+    "flat" (the per-query coarse search, LoadOptions::batched_coarse = false)."""
     from katoann import formats
     d = os.path.join(GOLDEN, name)
     ref = np.load(os.path.join(d, "reference_outputs.npz"))
@@ -54,9 +72,16 @@ def golden(name):
     cases = []
     for key in ref.files:
         if key.startswith("ids_"):
-            k, nprobe, S, ranking, selection = key[4:].split("_")
-            cases.append((int(k), int(nprobe), int(S), ranking, selection))
+            parts = key[4:].split("_")
+            k, nprobe, S, ranking, selection = parts[:5]
+            coarse = parts[5] if len(parts) > 5 else "batched"
+            cases.append((int(k), int(nprobe), int(S), ranking, selection, coarse))
     return os.path.join(d, "idx"), q, ref, sorted(cases)
+
+
+def case_tag(case):
+    k, nprobe, S, ranking, selection, coarse = case
+    return f"{k}_{nprobe}_{S}_{ranking}_{selection}" + ("_flat" if coarse == "flat" else "")
 
 
 def small_index(tmpdir, n=20000, nq=300, d=128, nlist=64, centres=64, seed=7, normalized=True, iters=5):

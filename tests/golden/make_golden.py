@@ -36,6 +36,42 @@ FIXTURES = {
 }
 
 
+# Per-query coarse search fixture: d 94 (the fp32 tail k >= 80 has 14 terms:
+# 8 rounded products added in order, then FMAs). Its centroids come in twins
+# that are equidistant from every query in exact arithmetic (coordinates
+# swapped between the sixteen chains, or permuted inside the tail), so the
+# reference's order of each twin pair is decided by the fp32 rounding alone.
+FLAT_FIXTURE = ("gflat", 1500, 160, 94, 32, 103)
+
+
+def crafted_flat_index(prefix, n, nq, d, nlist, seed):
+    rng = np.random.default_rng(seed)
+    half = nlist // 2
+    base_c = rng.standard_normal((half, d)).astype(np.float32) * 0.2
+    twins = base_c.copy()
+    for j in range(half):
+        if j % 2 == 0:  # swap coordinate pairs k, k + 1 (different chains) in the main part
+            for k in range(0, 80, 2):
+                twins[j, [k, k + 1]] = twins[j, [k + 1, k]]
+        else:  # permute the tail
+            twins[j, 80:] = twins[j, 80:][rng.permutation(d - 80)]
+    cen = np.empty((nlist, d), np.float32)
+    cen[0::2], cen[1::2] = base_c, twins
+    # queries symmetric in the swapped coordinates keep each twin pair
+    # equidistant in exact arithmetic
+    def sym(x):
+        x = x.copy()
+        x[:, 1:80:2] = x[:, 0:80:2]
+        x[:, 80:] = x[:, 80:81]
+        return x
+    q = sym(rng.standard_normal((nq, d)).astype(np.float32) * 0.2)
+    base = rng.standard_normal((n, d)).astype(np.float32) * 0.2
+    assign = ((base[:, None, :].astype(np.float64) - cen[None].astype(np.float64)) ** 2).sum(-1).argmin(1)
+    lists = [np.nonzero(assign == c)[0].astype(np.uint32) for c in range(nlist)]
+    formats.write_index(prefix, cen, lists, [base[ids] for ids in lists], n, False, None)
+    return q
+
+
 def grid(nlist):
     out = []
     for ranking in ("prefix", "high"):
@@ -47,7 +83,15 @@ def grid(nlist):
     for nprobe in (1, 4, min(16, nlist)):
         for k in (10, 32):
             out.append((k, nprobe, k, "full", "threshold"))
+    # per-query coarse search (LoadOptions::batched_coarse = false)
+    for (k, nprobe, S, ranking, selection) in ((10, 4, 32, "prefix", "threshold"), (10, nlist, 10, "full", "threshold"),
+                                               (32, min(16, nlist), 32, "high", "local"), (10, 1, 10, "prefix", "local")):
+        out.append((k, nprobe, S, ranking, selection, "flat"))
     return out
+
+
+def tag_of(case):
+    return "_".join(str(x) for x in case)
 
 
 def sha(a: np.ndarray) -> str:
@@ -75,15 +119,28 @@ def main():
         res = dict(lo=np.float32(R["lo"]), inv_step=np.float64(R["inv_step"]), step_sq=np.float64(R["step_sq"]),
                    prefix_dims=np.uint32(R["prefix_dims"]), codes_sha=sha(codes), ccodes_sha=sha(ccodes),
                    norm_sha=sha(R["norm"]), cnorm_sha=sha(R["cen_norm"]), rotated_sha=sha(rotated))
+        res["flat_order"] = O.ref_flat_order(prefix, os.path.join(out_dir, "queries.fbin"), nq, nlist)
         cases = grid(nlist)
-        for (k, nprobe, S, ranking, selection) in cases:
+        for case in cases:
+            k, nprobe, S, ranking, selection = case[:5]
+            coarse = case[5] if len(case) > 5 else "batched"
             ids, dists = O.ref_search(prefix, os.path.join(out_dir, "queries.fbin"), k, nprobe, S, ranking,
-                                      selection)
-            tag = f"{k}_{nprobe}_{S}_{ranking}_{selection}"
-            res["ids_" + tag] = ids.reshape(nq, k)
-            res["dists_" + tag] = dists.reshape(nq, k)
+                                      selection, coarse=coarse)
+            res["ids_" + tag_of(case)] = ids.reshape(nq, k)
+            res["dists_" + tag_of(case)] = dists.reshape(nq, k)
         np.savez_compressed(os.path.join(out_dir, "reference_outputs.npz"), **res)
         print(name, "cases", len(cases), "files", sorted(os.listdir(out_dir)))
+    name, n, nq, d, nlist, seed = FLAT_FIXTURE
+    out_dir = os.path.join(HERE, name)
+    if os.path.isdir(out_dir):
+        shutil.rmtree(out_dir)
+    os.makedirs(out_dir)
+    prefix = os.path.join(out_dir, "idx")
+    q = crafted_flat_index(prefix, n, nq, d, nlist, seed)
+    formats.write_fbin(os.path.join(out_dir, "queries.fbin"), q)
+    np.savez_compressed(os.path.join(out_dir, "reference_outputs.npz"),
+                        flat_order=O.ref_flat_order(prefix, os.path.join(out_dir, "queries.fbin"), nq, nlist))
+    print(name, "flat order", nq, "x", nlist)
 
 
 if __name__ == "__main__":

@@ -1,7 +1,8 @@
 """Ranking::kFull on the GPU (src/index.cpp:1130-1219): the exact 16-bit
 distance of every candidate of the probed lists, the k nearest by
-(distance, id); and the reference's equivalences between the three rankings
-and the two selections (tests/test_index.cpp:117-152).
+(distance, id); the per-query fp32 coarse search (LoadOptions::batched_coarse
+= false, src/index.cpp:832-844); and the reference's equivalences between the
+rankings, the selections and the coarse searches (tests/test_index.cpp:117-160).
 
 Rule: bit-exact ids and distances against the oracle on every query without a
 coarse tie at the nprobe-th list or a tie at the k-th exact distance (oracle
@@ -129,3 +130,42 @@ def test_reference_equivalences(decay_index):
     f16, _, _ = ix.search_ex(q, k=10, nprobe=16, ranking="full", normalize=False)
     r_full, r_prefix = recall(f16, gt), recall(ti, gt)
     assert r_full >= 0.9 and r_prefix >= r_full - 0.02
+    flat = load(prefix, batched_coarse=False)
+    assert not flat.batched_coarse and ix.batched_coarse
+    fl, _, _ = flat.search_ex(q, k=10, nprobe=16, normalize=False)
+    assert recall(fl, gt) >= r_prefix - 0.02
+
+
+@pytest.mark.parametrize("name", common.flat_golden_names())
+def test_flat_order_matches_reference(name):
+    """The per-query fp32 coarse search (batched_coarse = false) on the GPU
+    orders every list as the reference's coarse_flat_ does, and its distances
+    are the oracle's bit for bit."""
+    import os
+    from katoann import formats
+    d = os.path.join(common.GOLDEN, name)
+    ref = np.load(os.path.join(d, "reference_outputs.npz"))["flat_order"]
+    q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+    prefix = os.path.join(d, "idx")
+    ix = load(prefix, batched_coarse=False)
+    nl = ix.nlist
+    print("flat order", name, nl, flush=True)
+    _, _, ex = ix.search_ex(q, k=10, nprobe=nl, ranking="high", survivors=10, normalize=False,
+                            want=("probes", "probe_dists"))
+    assert np.array_equal(ex["probes"], ref)
+    D = O.OracleIndex.from_files(prefix).flat_dists(q)
+    assert np.array_equal(ex["probe_dists"].view(np.uint32),
+                          np.take_along_axis(D, ref.astype(np.int64), 1).view(np.uint32))
+
+
+@pytest.mark.parametrize("k,nprobe,ranking", [(10, 8, "prefix"), (10, 16, "full"), (32, 4, "high")])
+def test_flat_search_vs_oracle(decay_index, k, nprobe, ranking):
+    """Searches after the per-query coarse search equal the oracle's."""
+    prefix, q, _ = decay_index
+    ix = load(prefix, batched_coarse=False)
+    ox = O.OracleIndex.from_files(prefix).set_flat()
+    ids, dists, ex = ix.search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=32, normalize=False,
+                                  want=("probes",))
+    o = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=32)
+    assert np.array_equal(ex["probes"], o["probes"])
+    common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & common.tie_mask(ranking, "threshold")) != 0)

@@ -39,15 +39,16 @@ def test_load_matches_reference(name):
 @pytest.mark.parametrize("name", common.golden_names())
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
-    ix = load(prefix)
-    ox = O.OracleIndex.from_files(prefix)
+    ixs = {"batched": load(prefix), "flat": load(prefix, batched_coarse=False)}
+    oxs = {"batched": O.OracleIndex.from_files(prefix), "flat": O.OracleIndex.from_files(prefix).set_flat()}
     rows = 0
-    for (k, nprobe, S, ranking, selection) in cases:
-        tag = f"{k}_{nprobe}_{S}_{ranking}_{selection}"
+    for case in cases:
+        k, nprobe, S, ranking, selection, coarse = case
+        tag = common.case_tag(case)
         print("golden case", name, tag, flush=True)
-        ids, dists, _ = ix.search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, selection=selection,
-                                     normalize=False)
-        flags = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
+        ids, dists, _ = ixs[coarse].search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S,
+                                              selection=selection, normalize=False)
+        flags = oxs[coarse].search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
         tie = (flags & common.tie_mask(ranking, selection)) != 0
         n_ok, _ = common.compare(ids, dists, ref["ids_" + tag], ref["dists_" + tag], tie)
         rows += n_ok

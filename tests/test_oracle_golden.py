@@ -1,6 +1,7 @@
 """The CPU oracle against the reference's own outputs (golden fixtures written
 by tests/golden/make_golden.py from a build of the reference's sources)."""
 import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -32,10 +33,13 @@ def test_load_state_matches_reference(name):
 @pytest.mark.parametrize("name", common.golden_names())
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
-    ix = O.OracleIndex.from_files(prefix)
+    batched = O.OracleIndex.from_files(prefix)
+    flat = O.OracleIndex.from_files(prefix).set_flat()
     exact_rows = 0
-    for (k, nprobe, S, ranking, selection) in cases:
-        tag = f"{k}_{nprobe}_{S}_{ranking}_{selection}"
+    for case in cases:
+        k, nprobe, S, ranking, selection, coarse = case
+        tag = common.case_tag(case)
+        ix = flat if coarse == "flat" else batched
         r_ids, r_d = ref["ids_" + tag], ref["dists_" + tag]
         # libstdc++ tie emulation: identical to the reference on every row of kLocal
         o = ix.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, libstdc_ties=True)
@@ -48,3 +52,28 @@ def test_search_matches_reference(name):
         n_ok, _ = common.compare(c["ids"], c["dists"], r_ids, r_d, tie)
         exact_rows += n_ok
     assert exact_rows > 0
+
+
+@pytest.mark.parametrize("name", common.flat_golden_names())
+def test_flat_order_matches_reference(name):
+    """The per-query fp32 coarse distances order every list exactly as the
+    reference's coarse_flat_ does (index.cpp:832-844). The gflat fixture's
+    twin centroids are equidistant in exact arithmetic, so their order is
+    decided by the fp32 summation order alone."""
+    d = os.path.join(common.GOLDEN, name)
+    ref = np.load(os.path.join(d, "reference_outputs.npz"))["flat_order"]
+    q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+    D = O.OracleIndex.from_files(os.path.join(d, "idx")).flat_dists(q)
+    nl = D.shape[1]
+    order = np.stack([np.lexsort((np.arange(nl), D[i])) for i in range(len(q))])
+    assert np.array_equal(order, ref)
+
+
+def test_flat_dists_fp64_bound():
+    """The fp32 distances stay within a few ulps of the fp64 sum."""
+    d = os.path.join(common.GOLDEN, "g100")
+    q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+    raw = formats.read_index(os.path.join(d, "idx"))
+    D = O.OracleIndex.from_files(os.path.join(d, "idx")).flat_dists(q)
+    e = ((q[:, None, :].astype(np.float64) - raw["centroids"][None].astype(np.float64)) ** 2).sum(-1)
+    assert np.abs(D - e).max() <= 1e-5 * e.max()
