@@ -103,9 +103,12 @@ __global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, 
   uint64_t first, end;
   katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
   const uint64_t n = end - first;
-  const uint32_t tiles = static_cast<uint32_t>((n + rt - 1) / rt);
+  // the query chunks of one row tile are consecutive items: dealt to
+  // neighbouring waves (one CU, then the next block's), they read the tile's
+  // rows from HBM about once per L2 instead of once per chunk
+  const uint32_t nch = (gcnt[slot] + qc - 1) / qc;
   const uint32_t local = it - item_off[slot];
-  const uint32_t qch = local / tiles, rtile = local % tiles;
+  const uint32_t qch = local % nch, rtile = local / nch;
   uint4 d;
   d.x = qoff[slot] + qch * qc;
   d.y = min(qc, gcnt[slot] - qch * qc);
