@@ -45,7 +45,11 @@ typedef struct katoann_build_options {
   uint32_t jacobi_sweeps;
 } katoann_build_options;
 
-/* LoadOptions (index.h:37-43); use_amx is accepted and ignored. */
+/* LoadOptions (index.h:37-43); use_amx is accepted and ignored.
+ * precision KATOANN_BF16 keeps the unrotated vectors as bf16 (no codes, no
+ * .dimc read) and serves Ranking::kFull only (src/index.cpp:777-786, 885-889);
+ * batched_coarse = 0 selects the per-query fp32 coarse search
+ * (src/index.cpp:832-844), which bf16 precision always uses. */
 typedef struct katoann_load_options {
   int32_t precision;
   int32_t use_amx;

@@ -89,7 +89,7 @@ struct SearchCtx {
   hipEvent_t bev[2 * kMaxBands] = {};   // around each band's ranking kernel
   DevBuf q, y, qhi, qlo, qpre, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
       totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items, mx16, mh1,
-      mh2, mz;
+      mh2, mz, qbf, qbn;
   std::vector<uint32_t> host_totals;
   hipGraphExec_t exec = nullptr;  // the last captured pass and its launch arguments
   std::vector<uint64_t> gkey;
@@ -144,6 +144,9 @@ struct katoann_index {
 
   DevBuf rot, list_off_d, ids, id_row, hi, lo_plane, pre, norm, hinorm, headnorm, cen_hi, cen_lo, cen_norm;
   DevBuf cen_f;  // float centroids, unrotated: the per-query coarse search (batched = false)
+  // bf16 precision: unrotated rows as bf16, dp = roundup(dim, 32) apart, and their norms
+  DevBuf bf, bfn;
+  uint32_t dp = 0;
 
   std::mutex mu;
   std::vector<std::unique_ptr<SearchCtx>> free_ctx;
@@ -172,8 +175,9 @@ struct katoann_index {
     tile_prefix.clear();
     tile_prefix_rows = 0;
     for (DevBuf *b : {&rot, &list_off_d, &ids, &id_row, &hi, &lo_plane, &pre, &norm, &hinorm, &headnorm, &cen_hi,
-                      &cen_lo, &cen_norm, &cen_f})
+                      &cen_lo, &cen_norm, &cen_f, &bf, &bfn})
       b->release();
+    dp = 0;
   }
 
   std::unique_ptr<SearchCtx> acquire() const {
@@ -304,9 +308,7 @@ bool read_rotation(const std::string &prefix, const RawIndex &r, uint32_t *store
 // ---------------------------------------------------------------- load
 
 void check_load_options(const katoann_load_options &o) {
-  if (o.precision == KATOANN_BF16)
-    throw std::invalid_argument("bf16 precision is not supported by the gfx950 engine");
-  if (o.precision != KATOANN_SQ16) throw std::invalid_argument("precision is 'sq16' or 'bf16'");
+  if (o.precision != KATOANN_SQ16 && o.precision != KATOANN_BF16) throw std::invalid_argument("precision is 'sq16' or 'bf16'");
 }
 
 // Everything of Index::load after reading the files (src/index.cpp:777-807):
@@ -314,9 +316,11 @@ void check_load_options(const katoann_load_options &o) {
 // prefix plane, the centroid codes and the id map, all on the GPU.
 void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_t npts, bool normalized,
                         const float *centroids, const uint32_t *sizes, const uint32_t *ids, const float *vecs,
-                        bool vecs_on_device, const float *rotation, bool has_dimc, uint32_t dims, bool batched) {
+                        bool vecs_on_device, const float *rotation, bool has_dimc, uint32_t dims, bool batched,
+                        int32_t precision) {
   ix.reset();
   ix.batched = batched;
+  ix.precision = precision;
   ix.dim = dim;
   ix.nlist = nlist;
   ix.npts = npts;
@@ -327,6 +331,34 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
   for (uint32_t c = 0; c < nlist; ++c) ix.list_off[c + 1] = ix.list_off[c] + sizes[c];
   const uint64_t N = ix.list_off[nlist];
   ix.nrows = N;
+  if (precision == KATOANN_BF16) {
+    // src/index.cpp:777-786: the unrotated vectors as bf16 and their norms;
+    // no codes, no rotation, no prefix plane, no id map; the coarse search is
+    // the per-query one
+    ix.batched = false;
+    ix.dp = round_up(dim, 32u);
+    hipStream_t s = nullptr;
+    DevBuf raw;
+    const float *dv = vecs;
+    if (!vecs_on_device && N) {
+      upload(raw, vecs, static_cast<size_t>(N) * dim);
+      dv = raw.as<float>();
+    }
+    ix.bf.ensure(static_cast<size_t>(N) * ix.dp * 2);
+    ix.bfn.ensure(N * 4);
+    const uint64_t chunk = 1u << 22;
+    for (uint64_t r0 = 0; r0 < N; r0 += chunk) {
+      const uint32_t nr = static_cast<uint32_t>(std::min<uint64_t>(chunk, N - r0));
+      katoann_hip::launch_to_bf16(dv + r0 * dim, nr, dim, ix.dp, ix.bf.as<uint16_t>() + r0 * ix.dp,
+                                  ix.bfn.as<float>() + r0, s);
+    }
+    HIPCHECK(hipGetLastError());
+    upload(ix.cen_f, centroids, static_cast<size_t>(nlist) * dim);
+    upload(ix.ids, ids, N);
+    upload(ix.list_off_d, ix.list_off.data(), ix.list_off.size());
+    HIPCHECK(hipDeviceSynchronize());
+    return;
+  }
   if (has_dimc && dims) {
     if (dims % 64 || dims > ix.dc)
       throw std::invalid_argument("prefix_dims must be a multiple of 64 within the dimension");
@@ -439,12 +471,12 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
   uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
-  bool prefix, full, wave, ring;
+  bool prefix, full, bf16, wave, ring;
   uint64_t max_rows, allowance;
   size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
-      codes, npq, mx16, mh, pool;
+      codes, npq, mx16, mh, pool, qbf;
   size_t total() const {
-    return 2 * q + 2 * qcodes + qpre + qnorm + 2 * q4 + dist + 2 * probes + qlist + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
+    return 2 * q + 2 * qcodes + qpre + qbf + qnorm + 2 * q4 + dist + 2 * probes + qlist + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
            q4 /* pool_sorted */ + thr + q4 /* lbound */ + state + items + 2 * out + surv + (surv ? q4 : 0) + codes +
            npq + mx16 + 2 * mh + (mx16 ? q4 : 0) + pool;
   }
@@ -476,6 +508,8 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   if (ix.nlist == 0) throw logic_err("search before load");
   if (p.k == 0 || p.nprobe == 0 || p.nprobe > ix.nlist)
     throw std::invalid_argument("need k >= 1 and 1 <= nprobe <= nlist");
+  if (ix.precision == KATOANN_BF16 && p.ranking != KATOANN_RANK_FULL)
+    throw std::invalid_argument("bf16 precision supports only Ranking::kFull");
   if (p.ranking == KATOANN_RANK_PREFIX && ix.head_dims == 0)
     throw std::invalid_argument("prefix ranking needs the rotation file (.dimc)");
   if (p.ranking != KATOANN_RANK_PREFIX && p.ranking != KATOANN_RANK_HIGH && p.ranking != KATOANN_RANK_FULL)
@@ -484,6 +518,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
     throw std::invalid_argument("selection is 'threshold' or 'local'");
   SearchPlan z{};
   z.full = p.ranking == KATOANN_RANK_FULL;
+  z.bf16 = ix.precision == KATOANN_BF16;
   // full ranking keeps the k nearest of all candidates (src/index.cpp:1130-1219)
   z.S = z.full ? p.k : std::max(p.survivors, p.k);
   if (z.S > kMaxSurvivors)
@@ -502,7 +537,8 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.ring = !z.full && tc.kernel == 2;
   z.variant = z.full ? 0u : tc.variant;
   z.qc = z.wave || z.full ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
-  z.rt = z.full   ? katoann_hip::rank_full_rows()
+  z.rt = z.bf16   ? katoann_hip::rank_bf16_rows()
+         : z.full ? katoann_hip::rank_full_rows()
          : z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1)
                   : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
@@ -538,6 +574,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   // kernels gather them, and rows dc bytes apart would fall on a few L2
   // channels only
   z.qpre = z.prefix ? B * z.P : 0;
+  z.qbf = z.bf16 ? B * ix.dp * 2 + B * 4 : 0;
   z.qnorm = B * 8;
   z.q4 = B * 4;
   z.dist = B * z.nlist * 4;
@@ -586,6 +623,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   c.qhi.ensure(z.qcodes);
   c.qlo.ensure(z.qcodes);
   c.qpre.ensure(z.qpre);
+  if (z.bf16) {
+    c.qbf.ensure(static_cast<size_t>(B) * ix.dp * 2);
+    c.qbn.ensure(B * 4ull);
+  }
   c.qnorm.ensure(z.qnorm);
   c.qhn.ensure(z.q4);
   c.qh0.ensure(z.q4);
@@ -635,7 +676,8 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   // ranking blocks: as many as are resident at once
   const bool full = z.full;
   const uint32_t grid = static_cast<uint32_t>(device_cus()) *
-                        (full                ? katoann_hip::rank_full_blocks_per_cu()
+                        (z.bf16              ? katoann_hip::rank_bf16_blocks_per_cu()
+                         : full              ? katoann_hip::rank_full_blocks_per_cu()
                          : z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant)
                                              : 2u);
   uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
@@ -656,10 +698,13 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       katoann_hip::launch_rotate(c.q.as<float>(), B, d, ix.rot.as<float>(), c.y.as<float>(), s);
       yq = c.y.as<float>();
     }
-    katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
-                                 c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(),
-                                 c.qh0.as<int32_t>(), prefix ? c.qpre.as<int8_t>() : nullptr,
-                                 (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
+    if (z.bf16)  // src/index.cpp:1230-1237
+      katoann_hip::launch_to_bf16(c.q.as<float>(), B, d, ix.dp, c.qbf.as<uint16_t>(), c.qbn.as<float>(), s);
+    else
+      katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
+                                   c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(),
+                                   c.qh0.as<int32_t>(), prefix ? c.qpre.as<int8_t>() : nullptr,
+                                   (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
     mark(1);
     if (ix.batched)
       katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
@@ -707,7 +752,12 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const uint32_t qstride = prefix ? P : dc;
     for (uint32_t band = 0; band < nbands; ++band) {
       if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
-      if (full)
+      if (z.bf16)
+        katoann_hip::launch_rank_bf16(band, ix.bf.as<uint16_t>(), ix.bfn.as<float>(), ix.ids.as<uint32_t>(), nlist,
+                                      c.qbf.as<uint16_t>(), c.qbn.as<float>(), ix.dp, c.thr.as<uint64_t>(),
+                                      c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride,
+                                      c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
+      else if (full)
         katoann_hip::launch_rank_full(band, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
                                       ix.ids.as<uint32_t>(), nlist, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), dc,
                                       c.qnorm.as<int64_t>(), c.thr.as<uint64_t>(), c.qlist.as<uint32_t>(),
@@ -732,7 +782,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
         throw std::invalid_argument("unsupported survivor count");
     }
     mark(5);
-    if (!katoann_hip::launch_refine(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
+    if (z.bf16)
+      katoann_hip::launch_emit_bf16(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), B, K, dids, ddst,
+                                    (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
+                                    (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s);
+    else if (!katoann_hip::launch_refine(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
                                     B, S, K, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
                                     ix.id_row.as<uint32_t>(), ix.npts, dc, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(),
                                     c.qnorm.as<int64_t>(), ix.step_sq, dids, ddst,
@@ -750,11 +804,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     if (graphs) {
       // everything a launch reads: shapes, scalars, every buffer, the stream
       std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
-                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, full, grid, ix.gen,
+                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, full, z.bf16, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
                                    x && x->survivors, x && x->n_survivors, x && x->codes};
-      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qpre, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
+      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qpre, &c.qbf, &c.qbn, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
                               &c.npq, &c.gcnt, &c.qoff, &c.pscratch, &c.item_off, &c.qlist, &c.totals, &c.pool_cnt,
                               &c.pool_sorted, &c.thr, &c.lbound, &c.state, &c.pool, &c.surv, &c.nsurv, &c.codes,
                               &c.items, &c.mx16, &c.mh1, &c.mh2, &c.mz})
@@ -800,7 +854,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   // candidate; later searches of the shape start from the room that sufficed
   const std::array<uint64_t, 8> akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
                                         uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) |
-                                            (uint64_t(z.full) << 10)};
+                                            (uint64_t(z.full) << 10) | (uint64_t(z.bf16) << 11)};
   uint64_t allowance = z.allowance;
   {
     std::lock_guard<std::mutex> g(mix.tune_mu);
@@ -944,10 +998,12 @@ int katoann_load(katoann_index *h, const char *prefix_c, const katoann_load_opti
   uint32_t stored = 0;
   bool rotated = false;
   std::vector<float> rotation;
-  const bool has_dimc = read_rotation(prefix, r, &stored, &rotated, rotation);
+  // bf16 precision reads no .dimc (src/index.cpp:780-786)
+  const bool has_dimc = o.precision != KATOANN_BF16 && read_rotation(prefix, r, &stored, &rotated, rotation);
   const uint32_t dims = o.prefix_dims ? o.prefix_dims : stored;
   build_device_index(*h, r.dim, r.nlist, r.npts, r.normalized, r.centroids.data(), r.sizes.data(), r.ids.data(),
-                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims, o.batched_coarse != 0);
+                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims, o.batched_coarse != 0,
+                     o.precision);
   KATOANN_CATCH
 }
 
@@ -967,7 +1023,7 @@ int katoann_load_arrays(katoann_index *h, uint32_t dim, uint32_t nlist, uint32_t
   if (total != n) throw std::runtime_error("list sizes do not add up to n");
   const uint32_t dims = o.prefix_dims ? o.prefix_dims : prefix_dims;
   build_device_index(*h, dim, nlist, n, normalized != 0, centroids, list_sizes, ids, vectors,
-                     vectors_on_device != 0, rotation, true, dims, o.batched_coarse != 0);
+                     vectors_on_device != 0, rotation, true, dims, o.batched_coarse != 0, o.precision);
   KATOANN_CATCH
 }
 
@@ -1088,7 +1144,7 @@ int katoann_list_sizes(const katoann_index *h, uint32_t *out) {
 uint64_t katoann_memory_bytes(const katoann_index *h) {
   uint64_t s = 0;
   for (const DevBuf *b : {&h->rot, &h->list_off_d, &h->ids, &h->id_row, &h->hi, &h->lo_plane, &h->pre, &h->norm,
-                          &h->hinorm, &h->headnorm, &h->cen_hi, &h->cen_lo, &h->cen_norm, &h->cen_f})
+                          &h->hinorm, &h->headnorm, &h->cen_hi, &h->cen_lo, &h->cen_norm, &h->cen_f, &h->bf, &h->bfn})
     s += b->bytes;
   return s;
 }
@@ -1105,6 +1161,7 @@ int katoann_index_scale(const katoann_index *h, float *lo, double *inv_step, dou
 int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *centroid_codes, uint32_t *ids) {
   KATOANN_TRY
   if (h->nlist == 0) throw logic_err("search before load");
+  if (h->precision == KATOANN_BF16) throw std::invalid_argument("a bf16 index holds no 16-bit codes");
   const uint64_t N = h->nrows;
   const uint32_t dc = h->dc;
   auto unshift = [&](const DevBuf &hb, const DevBuf &lb, uint64_t rows, uint16_t *out) {

@@ -212,6 +212,37 @@ __global__ void k_rotate_tail(const float *__restrict__ X, uint32_t n, uint32_t 
 }
 
 // --------------------------------------------------------------------------
+// bf16 precision (src/index.cpp:752-775, 1230-1237; kernels.cpp:318-340): rows
+// rounded to bf16 (nearest even; the reference's _mm512_cvtneps_pbh and its
+// scalar tail agree on finite values), zero padded to dp = roundup(d, 32), and
+// their norms s = fmaf(v, v, s) in element order (the reference's scalar loop,
+// fused by its GCC build). One wave per row: lanes convert 64 elements at a
+// time, then every lane runs the same in-order chain over them.
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ X, uint32_t n, uint32_t d, uint32_t dp,
+                                                 uint16_t *__restrict__ out, float *__restrict__ norms) {
+  const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int lane = lane_id();
+  float s = 0.0f;
+  for (uint32_t k0 = 0; k0 < dp; k0 += 64) {
+    const uint32_t k = k0 + lane;
+    uint32_t b = 0;
+    if (k < d) {
+      const uint32_t u = __float_as_uint(X[static_cast<size_t>(row) * d + k]);
+      b = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    }
+    if (k < dp) out[static_cast<size_t>(row) * dp + k] = static_cast<uint16_t>(b);
+    const float v = __uint_as_float(b << 16);
+    const uint32_t m = min(64u, d > k0 ? d - k0 : 0u);
+    for (uint32_t l = 0; l < m; ++l) {
+      const float x = __shfl(v, static_cast<int>(l));
+      s = fmaf(x, x, s);
+    }
+  }
+  if (lane == 0) norms[row] = s;
+}
+
+// --------------------------------------------------------------------------
 // 16-bit codes (src/index.cpp:655-675, 917-943): v = clamp(lround((double(y) -
 // lo) * inv_step), 0, 65535), round half away from zero; NaN and |t| >= 2^63
 // give 0 like the reference's x86 conversion. One wave per row; writes the
@@ -1134,6 +1165,12 @@ void launch_coarse_flat(const float *Q, uint32_t B, uint32_t d, const float *cen
   if (!B || !nlist) return;
   dim3 grid((B + RT_ROWS - 1) / RT_ROWS, (nlist + RT_COLS - 1) / RT_COLS);
   hipLaunchKernelGGL(k_rotate<true>, grid, dim3(512), 0, s, Q, B, d, cen, nlist, dist, nlist);
+}
+
+void launch_to_bf16(const float *X, uint32_t n, uint32_t d, uint32_t dp, uint16_t *out, float *norms,
+                    hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_to_bf16, dim3((n + 3) / 4), dim3(256), 0, s, X, n, d, dp, out, norms);
 }
 
 void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32_t P, float lo,

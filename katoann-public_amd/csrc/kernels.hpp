@@ -105,6 +105,20 @@ void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const i
                       const uint64_t *thr, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
                       uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
                       uint32_t grid, hipStream_t s);
+// bf16 precision: rows to bf16 (dp apart, zero padded) and their norms;
+// ranking of one band in the reference's bf16 arithmetic (pools of the k
+// nearest, keys of order-preserving float bits); the k nearest out of the
+// pools.
+void launch_to_bf16(const float *X, uint32_t n, uint32_t d, uint32_t dp, uint16_t *out, float *norms,
+                    hipStream_t s);
+uint32_t rank_bf16_rows();
+uint32_t rank_bf16_blocks_per_cu();
+void launch_rank_bf16(uint32_t band, const uint16_t *rows, const float *xnorm, const uint32_t *ids, uint32_t nlist,
+                      const uint16_t *qbf, const float *qnorm, uint32_t dp, const uint64_t *thr,
+                      const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,
+                      uint64_t *pool, uint32_t *pool_cnt, uint32_t *state, uint32_t grid, hipStream_t s);
+void launch_emit_bf16(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, uint32_t B, uint32_t K,
+                      uint32_t *out_ids, float *out_dists, uint64_t *surv, uint32_t *nsurv, hipStream_t s);
 uint32_t refine_cap(uint32_t S);
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
                    uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s);

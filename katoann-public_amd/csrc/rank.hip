@@ -1062,6 +1062,123 @@ __global__ __launch_bounds__(256, 2) void k_rank_full(FullArgs a) {
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
+// --------------------------------------------------------------------------
+// Ranking in bf16 precision (Precision::kBf16, src/index.cpp:1221-1293 with
+// kernels.cpp:384-414), in the reference's AVX-512 arithmetic per
+// (query, row): eight fp32 accumulators, pair k2 (bf16 elements 2 k2 and
+// 2 k2 + 1) into accumulator k2 % 8 as _mm512_dpbf16_ps adds it (the odd
+// element's exact product first, then the even one's, each sum rounded), then
+// acc[g] += acc[g + s] for s = 4, 2, 1, and l2 = (qn + xn) - (dot + dot).
+// Lane = (accumulator g = lane % 8, row slot s = lane / 8): a wave ranks 8 rows
+// x 8 queries at a time, each lane running chain g of its row against the 8
+// queries (the 8 slots read the same query dwords: one access); the butterfly
+// over g (xor 4, 2, 1) is the reduction tree, after which lane (g, s) takes
+// query g x row s. Keys are order-preserving float bits << 32 | id (distances
+// can come out slightly negative); a key is kept iff below its query's
+// threshold key, so each pool ends with the k nearest by (distance, id).
+struct Bf16Args {
+  const uint32_t *rows;   // N x dp / 2 dwords (bf16 pairs)
+  const float *xnorm;     // N
+  const uint32_t *ids;    // N
+  const uint32_t *qbf;    // B x dp / 2
+  const float *qnorm;     // B
+  const uint64_t *thr;    // B
+  const uint32_t *qlist, *item_off;
+  const uint4 *items;
+  uint32_t nlist, band, dp2, stride;
+  uint64_t *pool;
+  uint32_t *pool_cnt, *state;
+};
+
+constexpr uint32_t BF16_ROWS = 32;  // rows per item
+
+__device__ __forceinline__ uint32_t ord_bits(float f) {
+  const uint32_t b = __float_as_uint(f == 0.0f ? 0.0f : f);  // -0 ranks as +0
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_ord(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ __launch_bounds__(256) void k_rank_bf16(Bf16Args a) {
+  const int lane = lane_id(), g = lane & 7, sl = lane >> 3;
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  const uint32_t nw = gridDim.x * 4;
+  uint32_t n_kept = 0;  // wave-uniform
+  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw) {
+    const uint4 d = a.items[it];
+    for (uint32_t r0 = d.z; r0 < d.w; r0 += 8) {
+      const uint32_t row = r0 + sl;
+      const bool rv = row < d.w;
+      const uint32_t rr = min(row, d.w - 1);
+      const uint32_t *xr = a.rows + static_cast<size_t>(rr) * a.dp2 + g;
+      const float xn = a.xnorm[rr];
+      const uint32_t xid = a.ids[rr];
+      for (uint32_t q0 = 0; q0 < d.y; q0 += 8) {
+        uint32_t qid[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) qid[i] = a.qlist[d.x + min(q0 + i, d.y - 1)];
+        float acc[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i] = 0.0f;
+        for (uint32_t k = 0; k < a.dp2; k += 8) {
+          const uint32_t xw = xr[k];
+          const float x0 = __uint_as_float(xw << 16), x1 = __uint_as_float(xw & 0xffff0000u);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const uint32_t qw = a.qbf[static_cast<size_t>(qid[i]) * a.dp2 + g + k];
+            acc[i] = fmaf(x1, __uint_as_float(qw & 0xffff0000u), acc[i]);
+            acc[i] = fmaf(x0, __uint_as_float(qw << 16), acc[i]);
+          }
+        }
+        float dot = 0.0f;
+        uint32_t q = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          float v = acc[i];
+          v = v + __shfl_xor(v, 4);
+          v = v + __shfl_xor(v, 2);
+          v = v + __shfl_xor(v, 1);
+          dot = g == i ? v : dot;
+          q = g == i ? qid[i] : q;
+        }
+        const float l2 = (a.qnorm[q] + xn) - (dot + dot);
+        const uint64_t key = (static_cast<uint64_t>(ord_bits(l2)) << 32) | xid;
+        const bool keep = rv && q0 + g < d.y && key < a.thr[q];
+        if (keep) {
+          const uint32_t pos = atomicAdd(&a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], 1u);
+          if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+          else atomicOr(&a.state[0], 1u);
+        }
+        n_kept += static_cast<uint32_t>(__popcll(ballot(keep)));
+      }
+    }
+  }
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
+// The k nearest of each pool after the last update (ascending, at most S = k
+// keys): ids and distances, padded with UINT32_MAX / FLT_MAX
+// (src/index.cpp:384-398); the survivors output as plain float bits << 32 | id.
+__global__ __launch_bounds__(64) void k_emit_bf16(const uint64_t *__restrict__ pool, uint32_t stride,
+                                                  const uint32_t *__restrict__ pool_cnt, uint32_t K,
+                                                  uint32_t *__restrict__ ids, float *__restrict__ dists,
+                                                  uint64_t *__restrict__ surv, uint32_t *__restrict__ nsurv) {
+  const uint32_t q = blockIdx.x;
+  const uint32_t n = min(pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], min(stride, K));
+  for (uint32_t i = threadIdx.x; i < K; i += 64) {
+    const bool ok = i < n;
+    const uint64_t key = ok ? pool[static_cast<size_t>(q) * stride + i] : ~0ull;
+    const float f = from_ord(static_cast<uint32_t>(key >> 32));
+    ids[static_cast<size_t>(q) * K + i] = ok ? static_cast<uint32_t>(key) : 0xffffffffu;
+    dists[static_cast<size_t>(q) * K + i] = ok ? f : FLT_MAX;
+    if (surv)
+      surv[static_cast<size_t>(q) * K + i] =
+          ok ? (static_cast<uint64_t>(__float_as_uint(f)) << 32) | static_cast<uint32_t>(key) : ~0ull;
+  }
+  if (nsurv && threadIdx.x == 0) nsurv[q] = n;
+}
+
 // After a band: each query's pool back to its S smallest keys, ascending, and
 // the query's threshold to the S-th of them (an upper bound of its final S-th
 // key), with the integer l2 bound of that threshold for the next band's
@@ -1559,6 +1676,24 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
 }
 
 uint32_t rank_full_rows() { return 16 * FULL_T; }
+uint32_t rank_bf16_rows() { return BF16_ROWS; }
+uint32_t rank_bf16_blocks_per_cu() { return 5; }  // 82 VGPRs: five waves per SIMD
+
+void launch_rank_bf16(uint32_t band, const uint16_t *rows, const float *xnorm, const uint32_t *ids, uint32_t nlist,
+                      const uint16_t *qbf, const float *qnorm, uint32_t dp, const uint64_t *thr,
+                      const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,
+                      uint64_t *pool, uint32_t *pool_cnt, uint32_t *state, uint32_t grid, hipStream_t s) {
+  const Bf16Args a{reinterpret_cast<const uint32_t *>(rows), xnorm, ids, reinterpret_cast<const uint32_t *>(qbf),
+                   qnorm, thr, qlist, item_off, items, nlist, band, dp / 2, stride, pool, pool_cnt, state};
+  hipLaunchKernelGGL(k_rank_bf16, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_emit_bf16(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, uint32_t B, uint32_t K,
+                      uint32_t *out_ids, float *out_dists, uint64_t *surv, uint32_t *nsurv, hipStream_t s) {
+  if (!B) return;
+  hipLaunchKernelGGL(k_emit_bf16, dim3(B), dim3(64), 0, s, pool, stride, pool_cnt, K, out_ids, out_dists, surv, nsurv);
+}
+
 uint32_t rank_full_blocks_per_cu() { return 3; }  // 142 VGPRs: three waves per SIMD
 
 void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const int64_t *xnorm, const uint32_t *ids,

@@ -63,7 +63,69 @@ struct OIndex {
   std::vector<uint32_t> id_row;    // npts -> row or UINT32_MAX
   std::vector<float> cen;          // nlist x d float centroids, unrotated (per-query coarse search)
   bool flat = false;               // LoadOptions::batched_coarse = false
+  // Precision::kBf16: unrotated vectors as bf16 rows dp = roundup(d, 32) apart, and their norms
+  uint32_t dp = 0;
+  std::vector<uint16_t> bf;
+  std::vector<float> bfn;
+  bool bf16 = false;
 };
+
+// ---- bf16 precision (index.cpp:752-775, 1221-1293; kernels.cpp:318-419)
+// to_bf16 (kernels.cpp:318-334): round to nearest even, as
+// _mm512_cvtneps_pbh and the scalar tail do for finite values.
+inline uint16_t bf16_of(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+inline float f32_of_bf16(uint16_t b) {
+  const uint32_t u = static_cast<uint32_t>(b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// bf16_norm (kernels.cpp:336-340): s += v * v in order; the GCC build fuses
+// each step (scalar FMAs, read from the assembly of the reference built here).
+float bf16_norm_o(const uint16_t *v, uint32_t d) {
+  float s = 0.0f;
+  for (uint32_t k = 0; k < d; ++k) {
+    const float x = f32_of_bf16(v[k]);
+    s = std::fmaf(x, x, s);
+  }
+  return s;
+}
+// bf16_l2, AVX-512 path (kernels.cpp:384-414): eight fp32 accumulators,
+// pair k2 (elements 2 k2, 2 k2 + 1) into accumulator k2 % 8 by
+// _mm512_dpbf16_ps: the odd element's product added first, then the even
+// one's, each product exact (8-bit significands) and each sum rounded; then
+// acc[g] += acc[g + s] for s = 4, 2, 1; l2 = (qn + xn) - (dot + dot).
+float bf16_dot(const uint16_t *q, const uint16_t *x, uint32_t dp) {
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t k2 = 0; k2 < dp / 2; ++k2) {
+    float &a = acc[k2 & 7];
+    const float p1 = f32_of_bf16(x[2 * k2 + 1]) * f32_of_bf16(q[2 * k2 + 1]);
+    const float p0 = f32_of_bf16(x[2 * k2]) * f32_of_bf16(q[2 * k2]);
+    a = a + p1;
+    a = a + p0;
+  }
+  for (int st = 4; st > 0; st >>= 1)
+    for (int g = 0; g < st; ++g) acc[g] = acc[g] + acc[g + st];
+  return acc[0];
+}
+// Order-preserving bits of a float (ties of -0 and +0 as +0), for (float, id) keys
+// of distances that may be negative.
+inline uint32_t ord_bits(float f) {
+  if (f == 0.0f) f = 0.0f;
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+inline float from_ord(uint32_t o) {
+  const uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
 
 // Reference: rotate_rows / rotate_tile4x4 / rotate_rows_serial
 // (index.cpp:139-270). For basis rows j < d & ~3: sixteen fmaf chains, chain l
@@ -263,6 +325,15 @@ void *oracle_index_from_raw(uint32_t d, uint32_t nlist, uint32_t npts, const uin
   const uint64_t N = ix->list_off[nlist];
   ix->ids.assign(ids, ids + N);
   ix->cen.assign(centroids, centroids + static_cast<size_t>(nlist) * d);
+  ix->dp = (d + 31) / 32 * 32;
+  ix->bf.assign(static_cast<size_t>(N) * ix->dp, 0);
+  ix->bfn.assign(N, 0.0f);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < static_cast<int64_t>(N); ++r) {
+    uint16_t *o = ix->bf.data() + static_cast<size_t>(r) * ix->dp;
+    for (uint32_t k = 0; k < d; ++k) o[k] = bf16_of(vecs[static_cast<size_t>(r) * d + k]);
+    ix->bfn[r] = bf16_norm_o(o, d);
+  }
   std::vector<float> rot(static_cast<size_t>(N) * d);
   std::vector<float> crot(static_cast<size_t>(nlist) * d);
   if (rotation) {
@@ -336,6 +407,16 @@ int oracle_index_set_flat(void *h, int flat) {
   auto *ix = static_cast<OIndex *>(h);
   if (flat && ix->cen.empty()) return 1;
   ix->flat = flat != 0;
+  return 0;
+}
+
+// bf16 precision (LoadOptions::precision = kBf16): full ranking over the bf16
+// rows, the per-query coarse search. Returns 0 on success.
+int oracle_index_set_bf16(void *h, int on) {
+  auto *ix = static_cast<OIndex *>(h);
+  if (on && ix->bf.empty()) return 1;
+  ix->bf16 = on != 0;
+  if (on) ix->flat = true;
   return 0;
 }
 
@@ -416,6 +497,7 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
   const uint32_t S = std::max(survivors, k);
   const uint32_t P = ranking == 2 ? ix->P : dc;
   if (ranking == 2 && P == 0) return 1;
+  if (ix->bf16 && ranking != 0) return 1;  // bf16 precision supports only Ranking::kFull
   std::vector<uint16_t> qc(static_cast<size_t>(nq) * dc);
   oracle_queries(h, queries, nq, nullptr, qc.data());
   SearchOut so{out_ids, out_dists, out_probes, out_probe_dists, out_surv, out_nsurv, out_flags};
@@ -477,9 +559,22 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
 
       // candidates of every probed list (index.cpp:998-1044)
       cand.clear();
+      std::vector<uint16_t> qb;
+      float qbn = 0.0f;
+      if (ix->bf16) {
+        qb.assign(ix->dp, 0);
+        for (uint32_t t = 0; t < d; ++t) qb[t] = bf16_of(queries[static_cast<size_t>(qi) * d + t]);
+        qbn = bf16_norm_o(qb.data(), d);
+      }
       for (uint32_t p = 0; p < npq; ++p) {
         const uint32_t c = ord[p];
         for (uint64_t r = ix->list_off[c]; r < ix->list_off[c + 1]; ++r) {
+          if (ix->bf16) {
+            const float dot = bf16_dot(qb.data(), ix->bf.data() + static_cast<size_t>(r) * ix->dp, ix->dp);
+            const float l2f = (qbn + ix->bfn[r]) - (dot + dot);
+            cand.emplace_back((static_cast<uint64_t>(ord_bits(l2f)) << 32) | ix->ids[r], static_cast<uint32_t>(r));
+            continue;
+          }
           const uint16_t *x = ix->codes.data() + static_cast<size_t>(r) * dc;
           int64_t l2;
           if (ranking == 0) {
@@ -506,7 +601,8 @@ int oracle_search(const void *h, const float *queries, uint32_t nq, uint32_t k, 
         for (size_t t = 0; t < m; ++t) {
           float f;
           const uint32_t b = static_cast<uint32_t>(cand[t].first >> 32);
-          std::memcpy(&f, &b, 4);
+          if (ix->bf16) f = from_ord(b);
+          else std::memcpy(&f, &b, 4);
           fin.emplace_back(f, static_cast<uint32_t>(cand[t].first));
         }
         if (so.nsurv) so.nsurv[qi] = 0;

@@ -57,6 +57,8 @@ def lib():
         L.oracle_normalize_rows.argtypes = [P, u64, u32]
         L.oracle_index_set_flat.restype = i32
         L.oracle_index_set_flat.argtypes = [P, i32]
+        L.oracle_index_set_bf16.restype = i32
+        L.oracle_index_set_bf16.argtypes = [P, i32]
         L.oracle_flat_dists.restype = i32
         L.oracle_flat_dists.argtypes = [P, P, u32, P]
         _lib = L
@@ -148,6 +150,13 @@ class OracleIndex:
         """Per-query fp32 coarse search (the reference's batched_coarse = false)."""
         if lib().oracle_index_set_flat(self.h, 1 if flat else 0) != 0:
             raise ValueError("the per-query coarse search needs the float centroids")
+        return self
+
+    def set_bf16(self, on=True):
+        """bf16 precision (the reference's LoadOptions::precision = kBf16):
+        full ranking over bf16 rows after the per-query coarse search."""
+        if lib().oracle_index_set_bf16(self.h, 1 if on else 0) != 0:
+            raise ValueError("bf16 precision needs an index from raw vectors")
         return self
 
     def flat_dists(self, q):

@@ -9,9 +9,10 @@
 //       rotation of every list (index.cpp:613-629)
 //   katoann_ref_search search PREFIX QUERIES.fbin OUT.bin k nprobe survivors
 //                             ranking(full|high|prefix) selection(threshold|local)
-//                             [batch=4096] [normalize=0] [coarse=batched|flat]
+//                             [batch=4096] [normalize=0] [mode=batched|flat|bf16]
 //       ids (u32) and dists (f32), nq x k, from Index::search in batches;
-//       coarse=flat loads with LoadOptions::batched_coarse = false
+//       flat loads with LoadOptions::batched_coarse = false, bf16 with
+//       LoadOptions::precision = kBf16
 //   katoann_ref_search flat   PREFIX QUERIES.fbin OUT.bin
 //       per query, every list in the order of the per-query coarse search
 //       (Index::coarse_flat_ with nprobe = nlist), u32, nq x nlist
@@ -131,7 +132,9 @@ int cmd_search(int argc, char **argv) {
   Index idx;
   katoann::LoadOptions lo;
   lo.use_amx = false;
-  lo.batched_coarse = !(argc > 12 && std::string(argv[12]) == "flat");
+  const std::string mode = argc > 12 ? argv[12] : "batched";
+  lo.batched_coarse = mode != "flat";
+  if (mode == "bf16") lo.precision = katoann::Precision::kBf16;
   idx.load(prefix, lo);
   uint32_t nq = 0, d = 0;
   std::vector<float> q = read_fbin(qpath, nq, d);

@@ -63,8 +63,9 @@ def flat_golden_names():
 
 def golden(name):
     """(index prefix, queries, reference outputs, cases); a case is
-    (k, nprobe, survivors, ranking, selection, coarse), coarse "batched" or
-    "flat" (the per-query coarse search, LoadOptions::batched_coarse = false)."""
+    (k, nprobe, survivors, ranking, selection, mode), mode "batched", "flat"
+    (the per-query coarse search, LoadOptions::batched_coarse = false) or
+    "bf16" (LoadOptions::precision = kBf16)."""
     from katoann import formats
     d = os.path.join(GOLDEN, name)
     ref = np.load(os.path.join(d, "reference_outputs.npz"))
@@ -81,7 +82,7 @@ def golden(name):
 
 def case_tag(case):
     k, nprobe, S, ranking, selection, coarse = case
-    return f"{k}_{nprobe}_{S}_{ranking}_{selection}" + ("_flat" if coarse == "flat" else "")
+    return f"{k}_{nprobe}_{S}_{ranking}_{selection}" + ("" if coarse == "batched" else "_" + coarse)
 
 
 def small_index(tmpdir, n=20000, nq=300, d=128, nlist=64, centres=64, seed=7, normalized=True, iters=5):

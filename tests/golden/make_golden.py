@@ -87,6 +87,10 @@ def grid(nlist):
     for (k, nprobe, S, ranking, selection) in ((10, 4, 32, "prefix", "threshold"), (10, nlist, 10, "full", "threshold"),
                                                (32, min(16, nlist), 32, "high", "local"), (10, 1, 10, "prefix", "local")):
         out.append((k, nprobe, S, ranking, selection, "flat"))
+    # bf16 precision (LoadOptions::precision = kBf16; full ranking only)
+    for nprobe in (1, 4, min(16, nlist)):
+        for k in (10, 32):
+            out.append((k, nprobe, k, "full", "threshold", "bf16"))
     return out
 
 
@@ -123,9 +127,9 @@ def main():
         cases = grid(nlist)
         for case in cases:
             k, nprobe, S, ranking, selection = case[:5]
-            coarse = case[5] if len(case) > 5 else "batched"
+            mode = case[5] if len(case) > 5 else "batched"
             ids, dists = O.ref_search(prefix, os.path.join(out_dir, "queries.fbin"), k, nprobe, S, ranking,
-                                      selection, coarse=coarse)
+                                      selection, coarse=mode)
             res["ids_" + tag_of(case)] = ids.reshape(nq, k)
             res["dists_" + tag_of(case)] = dists.reshape(nq, k)
         np.savez_compressed(os.path.join(out_dir, "reference_outputs.npz"), **res)

@@ -81,8 +81,6 @@ def test_dimc_checks(tmp_path):
     open(p + ".dimc", "wb").write(raw)
     with pytest.raises(ValueError, match="prefix_dims must be a multiple of 64"):
         ix.load(p, prefix_dims=96)
-    with pytest.raises(ValueError):
-        ix.load(p, precision="bf16")
     with pytest.raises(ValueError, match="precision is"):
         ix.load(p, precision="fp8")
 

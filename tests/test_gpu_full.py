@@ -169,3 +169,32 @@ def test_flat_search_vs_oracle(decay_index, k, nprobe, ranking):
     o = ox.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=32)
     assert np.array_equal(ex["probes"], o["probes"])
     common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & common.tie_mask(ranking, "threshold")) != 0)
+
+
+def check_bf16(ix, ox, q, k, nprobe):
+    ids, dists, ex = ix.search_ex(q, k=k, nprobe=nprobe, ranking="full", normalize=False, want=("probes",))
+    o = ox.search(q, k=k, nprobe=nprobe, ranking="full")
+    assert np.array_equal(ex["probes"], o["probes"])
+    return common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & common.tie_mask("full", "threshold")) != 0)
+
+
+@pytest.mark.parametrize("k,nprobe", [(10, 1), (10, 16), (32, 64), (100, 8)])
+def test_bf16_vs_oracle(decay_index, k, nprobe):
+    """bf16 precision (src/index.cpp:752-775, 1221-1293): per-query coarse
+    search, then every candidate's distance in the reference's bf16
+    arithmetic, the k nearest by (distance, id)."""
+    prefix, q, _ = decay_index
+    print("bf16 decay", k, nprobe, flush=True)
+    n_ok, _ = check_bf16(load(prefix, precision="bf16"), O.OracleIndex.from_files(prefix).set_bf16(), q, k, nprobe)
+    assert n_ok > 0
+
+
+def test_bf16_d1024_and_api(wide_index):
+    prefix, q = wide_index
+    ix = load(prefix, precision="bf16")
+    assert not ix.rotated and not ix.batched_coarse and ix.prefix_dims == 0
+    n_ok, _ = check_bf16(ix, O.OracleIndex.from_files(prefix).set_bf16(), q, 10, 12)
+    assert n_ok > 0
+    for ranking in ("prefix", "high"):
+        with pytest.raises(ValueError, match="bf16 precision supports only Ranking::kFull"):
+            ix.search(q[:4], k=10, nprobe=4, ranking=ranking)

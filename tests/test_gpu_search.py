@@ -39,8 +39,9 @@ def test_load_matches_reference(name):
 @pytest.mark.parametrize("name", common.golden_names())
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
-    ixs = {"batched": load(prefix), "flat": load(prefix, batched_coarse=False)}
-    oxs = {"batched": O.OracleIndex.from_files(prefix), "flat": O.OracleIndex.from_files(prefix).set_flat()}
+    ixs = {"batched": load(prefix), "flat": load(prefix, batched_coarse=False), "bf16": load(prefix, precision="bf16")}
+    oxs = {"batched": O.OracleIndex.from_files(prefix), "flat": O.OracleIndex.from_files(prefix).set_flat(),
+           "bf16": O.OracleIndex.from_files(prefix).set_bf16()}
     rows = 0
     for case in cases:
         k, nprobe, S, ranking, selection, coarse = case

@@ -34,12 +34,13 @@ def test_load_state_matches_reference(name):
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
     batched = O.OracleIndex.from_files(prefix)
-    flat = O.OracleIndex.from_files(prefix).set_flat()
+    oxs = {"batched": batched, "flat": O.OracleIndex.from_files(prefix).set_flat(),
+           "bf16": O.OracleIndex.from_files(prefix).set_bf16()}
     exact_rows = 0
     for case in cases:
-        k, nprobe, S, ranking, selection, coarse = case
+        k, nprobe, S, ranking, selection, mode = case
         tag = common.case_tag(case)
-        ix = flat if coarse == "flat" else batched
+        ix = oxs[mode]
         r_ids, r_d = ref["ids_" + tag], ref["dists_" + tag]
         # libstdc++ tie emulation: identical to the reference on every row of kLocal
         o = ix.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, libstdc_ties=True)
