@@ -90,6 +90,12 @@ int katoann_load(katoann_index *h, const char *prefix, const katoann_load_option
 int katoann_search(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
                    uint32_t *ids, float *dists);
 
+/* Index loads (no reference counterpart: the reference reads the whole
+ * .invlist into host memory, src/index.cpp:568-591) stream the vectors to the
+ * GPU through two pinned staging buffers of this many bytes each (default
+ * 64 MiB, 4 KiB to 4 GiB), after every file check has passed. Process-wide. */
+int katoann_set_load_chunk(uint64_t bytes);
+
 /* Accessors (index.h:83-93) */
 uint32_t katoann_dim(const katoann_index *h);
 uint32_t katoann_nlist(const katoann_index *h);

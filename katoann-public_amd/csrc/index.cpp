@@ -234,9 +234,54 @@ struct RawIndex {
   std::vector<uint32_t> sizes, ids;
 };
 
+// Pinned staging of the .invlist vectors on their way to device memory: two
+// buffers, the file read into one while the other's copy runs.
+std::atomic<uint64_t> g_load_chunk{64ull << 20};
+
+class Stager {
+ public:
+  explicit Stager(size_t bytes) : cap_(bytes) {
+    HIPCHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      HIPCHECK(hipHostMalloc(&buf_[i], cap_, hipHostMallocDefault));
+      HIPCHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+      HIPCHECK(hipEventRecord(ev_[i], s_));
+    }
+  }
+  ~Stager() {
+    (void)hipStreamSynchronize(s_);
+    for (int i = 0; i < 2; ++i) {
+      (void)hipHostFree(buf_[i]);
+      (void)hipEventDestroy(ev_[i]);
+    }
+    (void)hipStreamDestroy(s_);
+  }
+  size_t cap() const { return cap_; }
+  // the other buffer, once its last copy has finished
+  char *next() {
+    cur_ ^= 1;
+    HIPCHECK(hipEventSynchronize(ev_[cur_]));
+    return static_cast<char *>(buf_[cur_]);
+  }
+  void copy(void *dst, size_t n) {
+    HIPCHECK(hipMemcpyAsync(dst, buf_[cur_], n, hipMemcpyHostToDevice, s_));
+    HIPCHECK(hipEventRecord(ev_[cur_], s_));
+  }
+  void finish() { HIPCHECK(hipStreamSynchronize(s_)); }
+
+ private:
+  size_t cap_;
+  hipStream_t s_ = nullptr;
+  void *buf_[2] = {nullptr, nullptr};
+  hipEvent_t ev_[2] = {};
+  int cur_ = 0;
+};
+
 // Reads .meta, .centroids, .invlist with the reference's checks and messages
-// (src/index.cpp:550-591).
-void read_files(const std::string &prefix, RawIndex &r) {
+// (src/index.cpp:550-591). With `vec_off`, the vectors are not read: their
+// file offsets (one per list) are recorded after the same checks (the file
+// must hold every list's ids and vectors), for stream_vectors.
+void read_files(const std::string &prefix, RawIndex &r, std::vector<uint64_t> *vec_off = nullptr) {
   {
     std::ifstream f(prefix + ".meta", std::ios::binary);
     if (!f) throw std::runtime_error("cannot open " + prefix + ".meta");
@@ -268,6 +313,14 @@ void read_files(const std::string &prefix, RawIndex &r) {
   uint64_t total = 0;
   r.ids.clear();
   r.vecs.clear();
+  const uint64_t row_bytes = static_cast<uint64_t>(r.dim) * 4;
+  uint64_t fsize = 0;
+  if (vec_off) {
+    vec_off->assign(r.nlist, 0);
+    f.seekg(0, std::ios::end);
+    fsize = static_cast<uint64_t>(f.tellg());
+    f.seekg(0, std::ios::beg);
+  }
   for (uint32_t c = 0; c < r.nlist; ++c) {
     uint32_t sz = 0;
     f.read(reinterpret_cast<char *>(&sz), sizeof(sz));
@@ -276,12 +329,48 @@ void read_files(const std::string &prefix, RawIndex &r) {
     const size_t i0 = r.ids.size(), v0 = r.vecs.size();
     r.ids.resize(i0 + sz);
     f.read(reinterpret_cast<char *>(r.ids.data() + i0), static_cast<std::streamsize>(sz) * 4);
-    r.vecs.resize(v0 + static_cast<size_t>(sz) * r.dim);
-    f.read(reinterpret_cast<char *>(r.vecs.data() + v0), static_cast<std::streamsize>(sz) * r.dim * 4);
+    if (vec_off) {
+      if (!f) break;
+      const uint64_t at = static_cast<uint64_t>(f.tellg());
+      (*vec_off)[c] = at;
+      if (at + sz * row_bytes > fsize) {
+        f.setstate(std::ios::failbit);
+        break;
+      }
+      f.seekg(static_cast<std::streamoff>(at + sz * row_bytes), std::ios::beg);
+    } else {
+      r.vecs.resize(v0 + static_cast<size_t>(sz) * r.dim);
+      f.read(reinterpret_cast<char *>(r.vecs.data() + v0), static_cast<std::streamsize>(sz) * r.dim * 4);
+    }
     total += sz;
     if (!f) break;
   }
   if (!f || total != r.npts) throw std::runtime_error("truncated " + prefix + ".invlist");
+}
+
+// The vectors of a checked .invlist (offsets from read_files) into device
+// memory, npts x dim floats in list order, through the pinned stager: the host
+// never holds the whole file.
+void stream_vectors(const std::string &prefix, const RawIndex &r, const std::vector<uint64_t> &vec_off, DevBuf &dev) {
+  const uint64_t row_bytes = static_cast<uint64_t>(r.dim) * 4;
+  dev.ensure(std::max<uint64_t>(r.npts, 1) * row_bytes);
+  std::ifstream f(prefix + ".invlist", std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + prefix + ".invlist");
+  Stager st(static_cast<size_t>(std::max<uint64_t>(row_bytes, g_load_chunk.load())));
+  char *dst = dev.as<char>();
+  for (uint32_t c = 0; c < r.nlist; ++c) {
+    f.seekg(static_cast<std::streamoff>(vec_off[c]), std::ios::beg);
+    for (uint64_t left = r.sizes[c] * row_bytes; left > 0;) {
+      const size_t n = static_cast<size_t>(std::min<uint64_t>(left, st.cap()));
+      char *b = st.next();
+      f.read(b, static_cast<std::streamsize>(n));
+      if (!f) throw std::runtime_error("truncated " + prefix + ".invlist");
+      st.copy(dst, n);
+      dst += n;
+      left -= n;
+    }
+  }
+  st.finish();
 }
 
 // .dimc header and rotation (src/index.cpp:593-611); false when absent.
@@ -977,6 +1066,13 @@ void katoann_default_search_params(katoann_search_params *p) {
   *p = katoann_search_params{10u, 64u, KATOANN_RANK_PREFIX, 32u, KATOANN_SEL_THRESHOLD};
 }
 
+int katoann_set_load_chunk(uint64_t bytes) {
+  KATOANN_TRY
+  if (bytes < 4096 || bytes > (1ull << 32)) throw std::invalid_argument("load staging chunk must be 4 KiB to 4 GiB");
+  g_load_chunk.store(bytes);
+  KATOANN_CATCH
+}
+
 int katoann_index_create(katoann_index **out) {
   KATOANN_TRY
   *out = new katoann_index();
@@ -994,16 +1090,21 @@ int katoann_load(katoann_index *h, const char *prefix_c, const katoann_load_opti
   check_load_options(o);
   const std::string prefix(prefix_c);
   RawIndex r;
-  read_files(prefix, r);
+  std::vector<uint64_t> vec_off;
+  read_files(prefix, r, &vec_off);
   uint32_t stored = 0;
   bool rotated = false;
   std::vector<float> rotation;
   // bf16 precision reads no .dimc (src/index.cpp:780-786)
   const bool has_dimc = o.precision != KATOANN_BF16 && read_rotation(prefix, r, &stored, &rotated, rotation);
   const uint32_t dims = o.prefix_dims ? o.prefix_dims : stored;
+  if (has_dimc && dims && (dims % 64 || dims > round_up(r.dim, 64u)))
+    throw std::invalid_argument("prefix_dims must be a multiple of 64 within the dimension");
+  DevBuf dvecs;  // every file check has passed: the vectors go to device memory
+  stream_vectors(prefix, r, vec_off, dvecs);
   build_device_index(*h, r.dim, r.nlist, r.npts, r.normalized, r.centroids.data(), r.sizes.data(), r.ids.data(),
-                     r.vecs.data(), false, rotated ? rotation.data() : nullptr, has_dimc, dims, o.batched_coarse != 0,
-                     o.precision);
+                     dvecs.as<float>(), true, rotated ? rotation.data() : nullptr, has_dimc, dims,
+                     o.batched_coarse != 0, o.precision);
   KATOANN_CATCH
 }
 

@@ -118,6 +118,7 @@ def lib():
         L.katoann_set_pool_allowance.argtypes = [P, u32]
         L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
         L.katoann_set_graphs.argtypes = [P, i32]
+        L.katoann_set_load_chunk.argtypes = [u64]
         L.katoann_set_autotune.argtypes = [P, i32]
         L.katoann_tile_config.argtypes = [P, u32, C.POINTER(_SearchParams), P]
         L.katoann_sq16_l2.argtypes = [P, P, P, P, P, P, f64, u32, u32, u32, P]
@@ -166,6 +167,12 @@ def _f32(x, what="data must be a 2-D float32 array"):
     if a.ndim != 2:
         raise ValueError(what)
     return a
+
+
+def set_load_chunk(nbytes: int) -> None:
+    """Size of each of the two pinned staging buffers through which index
+    loads stream the .invlist vectors to the GPU (default 64 MiB)."""
+    _check(lib().katoann_set_load_chunk(int(nbytes)))
 
 
 def amx_available() -> bool:

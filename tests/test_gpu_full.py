@@ -198,3 +198,25 @@ def test_bf16_d1024_and_api(wide_index):
     for ranking in ("prefix", "high"):
         with pytest.raises(ValueError, match="bf16 precision supports only Ranking::kFull"):
             ix.search(q[:4], k=10, nprobe=4, ranking=ranking)
+
+
+def test_streamed_load_in_small_chunks(decay_index):
+    """The .invlist vectors stream to the GPU through pinned staging buffers;
+    with 64 KiB buffers (hundreds of chunks, lists split across them) the
+    loaded codes and the searches equal those of a load in one chunk."""
+    prefix, q, _ = decay_index
+    a = load(prefix)
+    try:
+        katoann.set_load_chunk(64 << 10)
+        b = load(prefix)
+        c = load(prefix, precision="bf16")
+    finally:
+        katoann.set_load_chunk(64 << 20)
+    ca, cca, _ = a.export_codes()
+    cb, ccb, _ = b.export_codes()
+    assert np.array_equal(ca, cb) and np.array_equal(cca, ccb)
+    ia, da, _ = a.search_ex(q, k=10, nprobe=16, normalize=False)
+    ib, db, _ = b.search_ex(q, k=10, nprobe=16, normalize=False)
+    assert np.array_equal(ia, ib) and np.array_equal(da, db)
+    n_ok, _ = check_bf16(c, O.OracleIndex.from_files(prefix).set_bf16(), q, 10, 8)
+    assert n_ok > 0
