@@ -608,6 +608,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   SearchPlan z{};
   z.full = p.ranking == KATOANN_RANK_FULL;
   z.bf16 = ix.precision == KATOANN_BF16;
+  if (z.bf16 && x && x->codes) throw std::invalid_argument("a bf16 index holds no 16-bit codes");
   // full ranking keeps the k nearest of all candidates (src/index.cpp:1130-1219)
   z.S = z.full ? p.k : std::max(p.survivors, p.k);
   if (z.S > kMaxSurvivors)
