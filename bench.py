@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--alpha", type=float, default=1.0)
     ap.add_argument("--nprobe", default=None)
     ap.add_argument("--survivors", default="32,128,256")
+    ap.add_argument("--extra-points", default=None,
+                    help="nprobe:survivors pairs added to the grid (default: 2048:512 on the 1M workload, where "
+                         "recall 0.999 needs 512 survivors)")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--kmeans-iters", type=int, default=10)
     ap.add_argument("--rank-rows", type=int, default=0)
@@ -154,6 +157,11 @@ def main():
     sweep = []
     grid = [(int(a), int(b)) for b in args.survivors.split(",") for a in args.nprobe.split(",")
             if int(a) <= args.nlist]
+    extra = args.extra_points if args.extra_points is not None else ("" if zipf else "2048:512")
+    for pt in filter(None, extra.split(",")):
+        xp, xs = (int(x) for x in pt.split(":"))
+        if xp <= args.nlist and (xp, xs) not in grid:
+            grid.append((xp, xs))
     # with several batches in flight, each host thread searches on its own
     # context (stream, buffers, captured graph) into its own outputs
     from concurrent.futures import ThreadPoolExecutor
