@@ -1437,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_refine(RefineArgs a) {
     const bool rok = sok && row != 0xffffffffu;
     v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
     // RB chunks of 64 bytes in flight at once: the rows are random gathers
-    constexpr int RB = 8;
+    constexpr int RB = 16;
     for (uint32_t kc0 = 0; kc0 < a.dc; kc0 += 64 * RB) {
       v4i xh[RB], xl[RB], bq[RB];
 #pragma unroll
