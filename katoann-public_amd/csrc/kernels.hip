@@ -55,7 +55,7 @@ constexpr int RT_K = 64;
 constexpr int RT_LD = RT_K + 2;  // 66 floats: the four slots of a wave hit disjoint banks
 
 template <bool L2>
-__global__ __launch_bounds__(512) void k_rotate(const float *__restrict__ X, uint32_t n, uint32_t d,
+__global__ __launch_bounds__(512, 4) void k_rotate(const float *__restrict__ X, uint32_t n, uint32_t d,
                                                 const float *__restrict__ R, uint32_t ncols,
                                                 float *__restrict__ Y, uint32_t ldy) {
   __shared__ float xs[2][RT_ROWS][RT_LD];

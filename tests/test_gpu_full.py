@@ -220,3 +220,24 @@ def test_streamed_load_in_small_chunks(decay_index):
     assert np.array_equal(ia, ib) and np.array_equal(da, db)
     n_ok, _ = check_bf16(c, O.OracleIndex.from_files(prefix).set_bf16(), q, 10, 8)
     assert n_ok > 0
+
+
+@pytest.mark.parametrize("mode", ["full", "flat", "bf16"])
+def test_per_query_nprobe_in_every_mode(decay_index, mode):
+    """Per-query nprobe (the learned-nprobe input) through full ranking, the
+    per-query coarse search and bf16 precision, against the oracle."""
+    prefix, q, _ = decay_index
+    rng = np.random.default_rng(3)
+    npq = rng.integers(1, 33, len(q)).astype(np.uint32)
+    if mode == "bf16":
+        ix, ox, ranking = load(prefix, precision="bf16"), O.OracleIndex.from_files(prefix).set_bf16(), "full"
+    elif mode == "flat":
+        ix, ox, ranking = load(prefix, batched_coarse=False), O.OracleIndex.from_files(prefix).set_flat(), "prefix"
+    else:
+        ix, ox, ranking = load(prefix), O.OracleIndex.from_files(prefix), "full"
+    print("per-query nprobe", mode, flush=True)
+    ids, dists, _ = ix.search_ex(q, k=10, nprobe=32, ranking=ranking, survivors=64, normalize=False,
+                                 nprobe_per_query=npq)
+    o = ox.search(q, k=10, nprobe=32, ranking=ranking, survivors=64, nprobe_per_query=npq)
+    n_ok, _ = common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & common.tie_mask(ranking, "threshold")) != 0)
+    assert n_ok > 0
