@@ -248,7 +248,7 @@ def read_ref_dump(path: str):
 
 def ref_search(prefix, queries_fbin, k, nprobe, survivors, ranking="prefix", selection="local",
                batch=4096, out=None, coarse="batched"):
-    """Runs the reference Index::search (AVX-512 engine) through the driver."""
+    """Runs the reference Index::search (AVX-512 engine) through katoann_ref_search."""
     import tempfile
     tmp = out or tempfile.mktemp(suffix=".bin")
     subprocess.run([REF_SEARCH, "search", prefix, queries_fbin, tmp, str(k), str(nprobe),
@@ -264,7 +264,7 @@ def ref_search(prefix, queries_fbin, k, nprobe, survivors, ranking="prefix", sel
 
 def ref_flat_order(prefix, queries_fbin, nq, nlist):
     """The reference's per-query coarse order of every list (Index::coarse_flat_
-    with nprobe = nlist) through the driver, nq x nlist."""
+    with nprobe = nlist) through katoann_ref_search, nq x nlist."""
     import tempfile
     tmp = tempfile.mktemp(suffix=".bin")
     subprocess.run([REF_SEARCH, "flat", prefix, queries_fbin, tmp], check=True)
