@@ -37,14 +37,14 @@ struct BuildOptions {
 
 enum class Precision {
   kSq16,  // 16-bit codes in high and low byte planes (default)
-  kBf16,  // not served by the gfx950 engine (load throws invalid_argument)
+  kBf16,  // bf16 rows, Ranking::kFull only (index.h:32-35)
 };
 
 struct LoadOptions {
   Precision precision = Precision::kSq16;
   bool use_amx = true;            // accepted for compatibility; the engine is the GPU
   uint32_t prefix_dims = 0;       // 0 takes the value stored in P.dimc
-  bool batched_coarse = true;     // the gfx950 engine always batches the coarse search
+  bool batched_coarse = true;     // false: the per-query fp32 coarse search
 };
 
 enum class Ranking { kFull, kHighBytes, kPrefix };

@@ -230,6 +230,11 @@ int katoann_sq16_l2(const uint8_t *qhi, const uint8_t *qlo, const uint8_t *xhi, 
 int katoann_sq16_high_l2(const uint8_t *qhi, const uint8_t *xhi, const int64_t *qnorm, const int64_t *xnorm,
                          double step_sq, uint32_t m, uint32_t n, uint32_t padded_dim, float *out,
                          uint32_t plane_dim);
+/* bf16 distances (reference kernels.h:64-67, bf16_l2; its AVX-512 arithmetic)
+ * on the GPU: m queries and n rows of dim bf16 values each, row-major (dim a
+ * multiple of 2), their norms; out[i][j] = (qnorm + xnorm) - 2 dot. */
+int katoann_bf16_l2(const uint16_t *q, const uint16_t *x, const float *qnorm, const float *xnorm, uint32_t m,
+                    uint32_t n, uint32_t dim, float *out);
 
 /* Kernels of the learned-nprobe MLP on device pointers (no reference
  * counterpart), on `stream` (hipStream_t, NULL for the null stream).

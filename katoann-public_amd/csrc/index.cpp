@@ -1463,6 +1463,24 @@ int katoann_sq16_high_l2(const uint8_t *qhi, const uint8_t *xhi, const int64_t *
   KATOANN_CATCH
 }
 
+int katoann_bf16_l2(const uint16_t *q, const uint16_t *x, const float *qnorm, const float *xnorm, uint32_t m,
+                    uint32_t n, uint32_t dim, float *out) {
+  KATOANN_TRY
+  if (m == 0 || n == 0) return KATOANN_OK;
+  if (dim == 0 || dim % 2) throw std::invalid_argument("dim must be a positive multiple of 2");
+  DevBuf a, b, c, e, o;
+  upload(a, q, static_cast<size_t>(m) * dim);
+  upload(b, x, static_cast<size_t>(n) * dim);
+  upload(c, qnorm, m);
+  upload(e, xnorm, n);
+  o.ensure(static_cast<size_t>(m) * n * 4);
+  katoann_hip::launch_bf16_pairs(a.as<uint16_t>(), b.as<uint16_t>(), c.as<float>(), e.as<float>(), m, n, dim,
+                                 o.as<float>(), nullptr);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out, o.p, static_cast<size_t>(m) * n * 4, hipMemcpyDeviceToHost));
+  KATOANN_CATCH
+}
+
 }  // extern "C"
 
 namespace {

@@ -242,6 +242,29 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ X, ui
   if (lane == 0) norms[row] = s;
 }
 
+// Kernel-level bf16 distances (kernels.cpp:384-414, row-major rows, dim a
+// multiple of 2): one thread per (query, row), the eight accumulators over
+// pairs k2 (k2 % 8), odd element first, then the 4/2/1 tree and
+// (qn + xn) - (dot + dot), as the ranking kernel computes them.
+__global__ void k_bf16_pairs(const uint32_t *__restrict__ q, const uint32_t *__restrict__ x,
+                             const float *__restrict__ qn, const float *__restrict__ xn, uint32_t m, uint32_t n,
+                             uint32_t d2, float *__restrict__ out) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= m * n) return;
+  const uint32_t i = t / n, j = t % n;
+  const uint32_t *qr = q + static_cast<size_t>(i) * d2, *xr = x + static_cast<size_t>(j) * d2;
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t k2 = 0; k2 < d2; ++k2) {
+    const uint32_t qw = qr[k2], xw = xr[k2];
+    float &a = acc[k2 & 7];
+    a = fmaf(__uint_as_float(xw & 0xffff0000u), __uint_as_float(qw & 0xffff0000u), a);
+    a = fmaf(__uint_as_float(xw << 16), __uint_as_float(qw << 16), a);
+  }
+  for (int st = 4; st > 0; st >>= 1)
+    for (int g = 0; g < st; ++g) acc[g] = acc[g] + acc[g + st];
+  out[static_cast<size_t>(i) * n + j] = (qn[i] + xn[j]) - (acc[0] + acc[0]);
+}
+
 // --------------------------------------------------------------------------
 // 16-bit codes (src/index.cpp:655-675, 917-943): v = clamp(lround((double(y) -
 // lo) * inv_step), 0, 65535), round half away from zero; NaN and |t| >= 2^63
@@ -1165,6 +1188,15 @@ void launch_coarse_flat(const float *Q, uint32_t B, uint32_t d, const float *cen
   if (!B || !nlist) return;
   dim3 grid((B + RT_ROWS - 1) / RT_ROWS, (nlist + RT_COLS - 1) / RT_COLS);
   hipLaunchKernelGGL(k_rotate<true>, grid, dim3(512), 0, s, Q, B, d, cen, nlist, dist, nlist);
+}
+
+void launch_bf16_pairs(const uint16_t *q, const uint16_t *x, const float *qn, const float *xn, uint32_t m,
+                       uint32_t n, uint32_t dim, float *out, hipStream_t s) {
+  const uint64_t t = static_cast<uint64_t>(m) * n;
+  if (!t) return;
+  hipLaunchKernelGGL(k_bf16_pairs, dim3(static_cast<uint32_t>((t + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const uint32_t *>(q), reinterpret_cast<const uint32_t *>(x), qn, xn, m, n,
+                     dim / 2, out);
 }
 
 void launch_to_bf16(const float *X, uint32_t n, uint32_t d, uint32_t dp, uint16_t *out, float *norms,

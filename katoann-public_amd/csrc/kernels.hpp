@@ -111,6 +111,8 @@ void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const i
 // pools.
 void launch_to_bf16(const float *X, uint32_t n, uint32_t d, uint32_t dp, uint16_t *out, float *norms,
                     hipStream_t s);
+void launch_bf16_pairs(const uint16_t *q, const uint16_t *x, const float *qn, const float *xn, uint32_t m,
+                       uint32_t n, uint32_t dim, float *out, hipStream_t s);
 uint32_t rank_bf16_rows();
 uint32_t rank_bf16_blocks_per_cu();
 void launch_rank_bf16(uint32_t band, const uint16_t *rows, const float *xnorm, const uint32_t *ids, uint32_t nlist,
