@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
                                                     uint32_t np, uint32_t *__restrict__ probes,
                                                     float *__restrict__ probe_d) {
   extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x max(M, SEL_FAST if np <= 256)
-  __shared__ uint32_t hist[4][SEL_BINS];
+  __shared__ uint32_t hist[SEL_BINS];  // one for the four waves: LDS for four blocks per CU
   __shared__ uint32_t tmp[8], sh_digit, sh_rank;
   const uint32_t q = blockIdx.x;
   const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
@@ -729,12 +729,12 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
   for (int top = nbits; top > 0; top -= 11) {
     const int shift = max(0, top - 11);
     const uint32_t dmask = (1u << (top - shift)) - 1u;
-    for (int i = tid; i < 4 * SEL_BINS; i += 256) (&hist[0][0])[i] = 0;
+    for (int i = tid; i < SEL_BINS; i += 256) hist[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const uint32_t r = v[i] - vmin;
-      if (c0 + i < nlist && (r & mask) == prefix) atomicAdd(&hist[w][(r >> shift) & dmask], 1u);
+      if (c0 + i < nlist && (r & mask) == prefix) atomicAdd(&hist[(r >> shift) & dmask], 1u);
     }
     __syncthreads();
     // bins 8 tid .. 8 tid + 7, summed over the waves; the bin holding the rank-th
@@ -742,7 +742,7 @@ __global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ di
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int b = 8 * tid + i;
-      h[i] = hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
+      h[i] = hist[b];
       sum += h[i];
     }
     uint32_t cum = block_exscan256(sum, tmp);
