@@ -626,7 +626,7 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
 constexpr uint32_t SEL_FAST = 1024;  // candidates of the small-np path at most
 
 template <int V>
-__global__ __launch_bounds__(256) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
+__global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
                                                     uint32_t np, uint32_t *__restrict__ probes,
                                                     float *__restrict__ probe_d) {
   extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x max(M, SEL_FAST if np <= 256)
