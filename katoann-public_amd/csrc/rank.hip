@@ -1283,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
 }
 
 template <int SC>
-__global__ __launch_bounds__(256) void k_update(UpdateArgs a) {
+__global__ __launch_bounds__(256, (SC == 512 ? 5 : 1)) void k_update(UpdateArgs a) {
   __shared__ uint64_t part[3][SC];
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x;
