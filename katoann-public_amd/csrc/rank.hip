@@ -1385,7 +1385,7 @@ struct RefineArgs {
 //  4. output ascending by (distance, id), padded with FLT_MAX / UINT32_MAX
 //     (src/index.cpp:384-398).
 template <int SC>
-__global__ __launch_bounds__(256) void k_refine(RefineArgs a) {
+__global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   __shared__ uint64_t ex[4][SC];
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x * 4 + w;
