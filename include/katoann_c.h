@@ -145,6 +145,9 @@ typedef struct katoann_search_extras {
   const katoann_nprobe_model *nprobe_model; /* per-query nprobe from the model (exclusive with nprobe_per_query) */
   uint32_t *nprobe_used;            /* nq: the per-query nprobe the search used */
   int32_t coarse_only;              /* stop after the probe selection (probes / probe_dists only) */
+  uint64_t max_bytes;               /* nonzero: an overflow rerun whose buffers would exceed this many bytes
+                                       stops with KATOANN_RUNTIME_ERROR (and *needed_bytes set) instead */
+  uint64_t *needed_bytes;           /* out: device bytes of the search's buffers (or of the refused rerun) */
 } katoann_search_extras;
 
 int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
@@ -289,6 +292,7 @@ typedef struct katoann_server_stats {
   double achieved_qps;                 /* queries / (last completion - first arrival) */
   uint64_t budget, peak_reserved;      /* bytes */
   uint32_t peak_inflight;
+  uint64_t peak_used;                  /* bytes: the largest search buffers a batch ran with */
 } katoann_server_stats;
 
 void katoann_default_server_options(katoann_server_options *o);

@@ -23,6 +23,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/katoann_c.h"
@@ -33,7 +34,7 @@ namespace {
 constexpr uint32_t kMaxDim = 4096;          // kernels.h:24
 constexpr uint32_t kDimcMagic = 0x434D4944u;  // index.cpp:27
 constexpr uint32_t kDimcVersion = 2u;       // index.cpp:28
-constexpr uint32_t kMaxSurvivors = 1024;
+constexpr uint32_t kMaxSurvivors = 4096;
 constexpr uint32_t kPoolAllowance = 1024;  // pool slots per query beyond S for one band's kept keys
 constexpr uint64_t kPoolAllowanceCap = 65536;  // ... at most, unless one list is larger (overflow reruns)
 constexpr uint32_t kMaxProbes = 4096;
@@ -87,10 +88,7 @@ struct SearchCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[KATOANN_NSTAGES] = {};  // stage boundaries (the pool updates are timed as the rest of the ranking interval)
   hipEvent_t bev[2 * kMaxBands] = {};   // around each band's ranking kernel
-  DevBuf q, y, qhi, qlo, qpre, qnorm, qhn, qh0, dist, probes, probe_d, npq, gcnt, qoff, pscratch, item_off, qlist,
-      totals, pool_cnt, pool_sorted, thr, lbound, state, pool, ids, dists, surv, nsurv, codes, items, mx16, mh1,
-      mh2, mz, qbf, qbn;
-  std::vector<uint32_t> host_totals;
+  DevBuf ws;                            // the search's buffers, carved from one allocation
   hipGraphExec_t exec = nullptr;  // the last captured pass and its launch arguments
   std::vector<uint64_t> gkey;
   SearchCtx() {
@@ -445,6 +443,7 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
     upload(ix.cen_f, centroids, static_cast<size_t>(nlist) * dim);
     upload(ix.ids, ids, N);
     upload(ix.list_off_d, ix.list_off.data(), ix.list_off.size());
+    (void)ix.max_rows_per_query(1);
     HIPCHECK(hipDeviceSynchronize());
     return;
   }
@@ -551,6 +550,7 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
     upload(ix.id_row, idr.data(), idr.size());
   }
   upload(ix.list_off_d, ix.list_off.data(), ix.list_off.size());
+  (void)ix.max_rows_per_query(1);  // the largest lists' sums, ahead of any search
   HIPCHECK(hipDeviceSynchronize());
 }
 
@@ -563,13 +563,92 @@ struct SearchPlan {
   bool prefix, full, bf16, wave, ring;
   uint64_t max_rows, allowance;
   size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
-      codes, npq, mx16, mh, pool, qbf;
-  size_t total() const {
-    return 2 * q + 2 * qcodes + qpre + qbf + qnorm + 2 * q4 + dist + 2 * probes + qlist + 2 * gcnt + pscratch + item_off + 16 + pool_cnt +
-           q4 /* pool_sorted */ + thr + q4 /* lbound */ + state + items + 2 * out + surv + (surv ? q4 : 0) + codes +
-           npq + mx16 + 2 * mh + (mx16 ? q4 : 0) + pool;
-  }
+      codes, npq, mx16, mh, qbf, nsurv;
+  std::array<uint64_t, 8> akey;  // shape and tile: the key of the pool room an overflow rerun needed
 };
+
+// Device buffers of one search, carved in this order from one allocation
+// (every buffer 256-byte aligned; the pool, whose room varies, last).
+struct Workspace {
+  float *q, *y;
+  int8_t *qhi, *qlo, *qpre;
+  uint16_t *qbf;
+  float *qbn;
+  int64_t *qnorm;
+  int32_t *qhn, *qh0;
+  float *dist;
+  uint32_t *probes;
+  float *probe_d;
+  uint32_t *npq, *gcnt, *qoff, *pscratch, *item_off, *qlist, *totals, *pool_cnt, *pool_sorted;
+  uint64_t *thr;
+  int32_t *lbound;
+  uint32_t *state;
+  uint4 *items;
+  uint32_t *ids;
+  float *dists;
+  uint64_t *surv;
+  uint32_t *nsurv;
+  uint16_t *codes;
+  void *mx16, *mh1, *mh2;
+  float *mz;
+  uint64_t *pool;
+};
+
+// Bytes of the workspace of plan z with `stride` pool slots per query; with
+// base, also the buffers' addresses.
+size_t carve(const SearchPlan &z, uint64_t stride, char *base, Workspace *w) {
+  size_t off = 0;
+  auto take = [&](auto &ptr, size_t bytes) {
+    using Ptr = std::remove_reference_t<decltype(ptr)>;
+    ptr = bytes && base ? reinterpret_cast<Ptr>(base + off) : nullptr;
+    off += (bytes + 255) / 256 * 256;
+  };
+  Workspace tmp{};
+  Workspace &v = w ? *w : tmp;
+  const size_t B = z.B;
+  take(v.q, z.q);
+  take(v.y, z.q);
+  take(v.qhi, z.qcodes);
+  take(v.qlo, z.qcodes);
+  take(v.qpre, z.qpre);
+  take(v.qbf, z.qbf);
+  take(v.qbn, z.qbf ? B * 4 : 0);
+  take(v.qnorm, z.qnorm);
+  take(v.qhn, z.q4);
+  take(v.qh0, z.q4);
+  take(v.dist, z.dist);
+  take(v.probes, z.probes);
+  take(v.probe_d, z.probes);
+  take(v.npq, z.npq);
+  take(v.gcnt, z.gcnt);
+  take(v.qoff, z.gcnt);
+  take(v.pscratch, z.pscratch);
+  take(v.item_off, z.item_off);
+  take(v.qlist, z.qlist);
+  take(v.totals, 16);
+  take(v.pool_cnt, z.pool_cnt);
+  take(v.pool_sorted, z.q4);
+  take(v.thr, z.thr);
+  take(v.lbound, z.q4);
+  take(v.state, z.state);
+  take(v.items, z.items);
+  take(v.ids, z.out);
+  take(v.dists, z.out);
+  take(v.surv, z.surv);
+  take(v.nsurv, z.nsurv);
+  take(v.codes, z.codes);
+  take(v.mx16, z.mx16);
+  take(v.mh1, z.mh);
+  take(v.mh2, z.mh);
+  take(v.mz, z.mx16 ? z.q4 : 0);
+  take(v.pool, std::max<uint64_t>(stride * B, 1) * 8);
+  return off;
+}
+
+// Pool slots per query of plan z with room `allowance` beyond S.
+uint64_t pool_stride(const SearchPlan &z, uint64_t allowance) {
+  return std::min<uint64_t>(z.S + allowance, 0xffffffffu);
+}
 
 TileKey tile_key(const katoann_index &ix, uint32_t nq, const katoann_search_params &p) {
   const bool prefix = p.ranking == KATOANN_RANK_PREFIX;
@@ -612,7 +691,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   // full ranking keeps the k nearest of all candidates (src/index.cpp:1130-1219)
   z.S = z.full ? p.k : std::max(p.survivors, p.k);
   if (z.S > kMaxSurvivors)
-    throw std::invalid_argument("max(survivors, k) above 1024 is not supported by the gfx950 engine");
+    throw std::invalid_argument("max(survivors, k) above 4096 is not supported by the gfx950 engine");
   if (p.nprobe > kMaxProbes) throw std::invalid_argument("nprobe above 4096 is not supported by the gfx950 engine");
   z.prefix = p.ranking == KATOANN_RANK_PREFIX;
   z.P = z.prefix ? ix.head_dims : ix.dc;
@@ -636,8 +715,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.nbands = katoann_hip::plan_bands(z.np, z.shift, z.seed);
   if (z.nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
   z.nstate = 3 + z.nbands;
-  auto &mix = const_cast<katoann_index &>(ix);
-  z.max_rows = mix.max_rows_per_query(z.np);
+  z.max_rows = const_cast<katoann_index &>(ix).max_rows_per_query(z.np);
   // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
   // groups over all bands hold at most B queries
   uint64_t tiles_total = 0;
@@ -653,10 +731,21 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.allowance =
       z.seed ? std::min<uint64_t>(z.max_rows, std::max<uint64_t>(kPoolAllowance, 2ull * z.seed))
              : std::min(z.max_rows,
-                        std::max<uint64_t>({kPoolAllowance, 2 * mix.max_rows_per_query(1),
+                        std::max<uint64_t>({kPoolAllowance, 2 * const_cast<katoann_index &>(ix).max_rows_per_query(1),
                                             std::min(kPoolAllowanceCap,
-                                                     mix.max_rows_per_query(std::min(z.np, 1u + (1u << z.shift))))}));
+                                                     const_cast<katoann_index &>(ix).max_rows_per_query(
+                                                         std::min(z.np, 1u + (1u << z.shift))))}));
   if (ix.pool_allowance) z.allowance = std::min<uint64_t>(z.max_rows, ix.pool_allowance);
+  z.akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
+            uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) | (uint64_t(z.full) << 10) |
+                (uint64_t(z.bf16) << 11)};
+  {
+    // later searches of a shape start from the room its last overflow rerun needed
+    auto &mix = const_cast<katoann_index &>(ix);
+    std::lock_guard<std::mutex> g(mix.tune_mu);
+    auto it = mix.allowance_seen.find(z.akey);
+    if (it != mix.allowance_seen.end()) z.allowance = std::max(z.allowance, it->second);
+  }
   const size_t B = z.B;
   z.q = B * z.d * 4;
   z.qcodes = B * z.dc;
@@ -664,7 +753,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   // kernels gather them, and rows dc bytes apart would fall on a few L2
   // channels only
   z.qpre = z.prefix ? B * z.P : 0;
-  z.qbf = z.bf16 ? B * ix.dp * 2 + B * 4 : 0;
+  z.qbf = z.bf16 ? B * ix.dp * 2 : 0;
   z.qnorm = B * 8;
   z.q4 = B * 4;
   z.dist = B * z.nlist * 4;
@@ -680,6 +769,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.items = static_cast<size_t>(z.item_cap) * 16;
   z.out = B * z.K * 4;
   z.surv = (x && x->survivors) ? B * z.S * 8 : 0;
+  z.nsurv = (x && x->n_survivors) ? B * 4 : 0;
   z.codes = (x && x->codes) ? B * z.dc * 2 : 0;
   const katoann_nprobe_model *model = x ? x->nprobe_model : nullptr;
   if (model) {
@@ -690,79 +780,44 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
     z.mh = B * model->hidden * 2;
   }
   z.npq = (model || (x && x->nprobe_per_query)) ? B * 4 : 0;
-  z.pool = std::max<uint64_t>(static_cast<uint64_t>(std::min<uint64_t>(z.S + z.allowance, 0xffffffffu)) * B, 1) * 8;
   return z;
 }
 
+// Device memory a search runs in instead of buffers of its own (the
+// allocation-free entry point): base, bytes.
+struct FixedScratch {
+  char *base;
+  uint64_t bytes;
+};
+
 void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, const katoann_search_params &p,
                  const katoann_search_extras *x, uint32_t *out_ids, float *out_dists,
-                 const TileConfig *force = nullptr) {
+                 const TileConfig *force = nullptr, const FixedScratch *fixed = nullptr) {
   if (nq == 0) return;
   const SearchPlan z = plan_search(ix, nq, p, x, force);
   const uint32_t B = z.B, d = z.d, dc = z.dc, np = z.np, K = z.K, nlist = z.nlist, S = z.S, P = z.P;
   const bool prefix = z.prefix;
   const bool dev_io = x && x->device_io;
+  if (fixed && !dev_io) throw std::invalid_argument("the allocation-free search takes device pointers (device_io)");
   auto &mix = const_cast<katoann_index &>(ix);
-  auto ctx = ix.acquire();
-  SearchCtx &c = *ctx;
-  hipStream_t s = (dev_io && x->stream) ? static_cast<hipStream_t>(x->stream) : c.stream;
-  const bool timing = ix.timing.load();
-
-  c.q.ensure(z.q);
-  c.y.ensure(z.q);
-  c.qhi.ensure(z.qcodes);
-  c.qlo.ensure(z.qcodes);
-  c.qpre.ensure(z.qpre);
-  if (z.bf16) {
-    c.qbf.ensure(static_cast<size_t>(B) * ix.dp * 2);
-    c.qbn.ensure(B * 4ull);
-  }
-  c.qnorm.ensure(z.qnorm);
-  c.qhn.ensure(z.q4);
-  c.qh0.ensure(z.q4);
-  c.dist.ensure(z.dist);
-  c.probes.ensure(z.probes);
-  c.probe_d.ensure(z.probes);
-  const uint32_t shift = z.shift, qc = z.qc, nslots = z.nslots, nbands = z.nbands, nstate = z.nstate, rt = z.rt;
-  c.gcnt.ensure(z.gcnt);
-  c.qoff.ensure(z.gcnt);
-  c.pscratch.ensure(z.pscratch);
-  c.item_off.ensure(z.item_off);
-  c.qlist.ensure(z.qlist);
-  c.totals.ensure(16);
-  c.pool_cnt.ensure(z.pool_cnt);
-  c.pool_sorted.ensure(z.q4);
-  c.thr.ensure(z.thr);
-  c.lbound.ensure(z.q4);
-  c.state.ensure(z.state);
-  const uint64_t max_rows = z.max_rows;
-  const uint32_t item_cap = z.item_cap;
-  c.items.ensure(z.items);
-  c.ids.ensure(z.out);
-  c.dists.ensure(z.out);
-  if (z.surv) c.surv.ensure(z.surv);
-  if (x && x->n_survivors) c.nsurv.ensure(z.q4);
-  if (z.codes) c.codes.ensure(z.codes);
+  // the allocation-free search runs on the caller's stream with no context
+  // (a context owns a stream, events and buffers of its own)
+  std::unique_ptr<SearchCtx> ctx = fixed ? nullptr : ix.acquire();
+  struct Release {  // the context goes back to the pool on every exit
+    const katoann_index &ix;
+    std::unique_ptr<SearchCtx> &c;
+    ~Release() {
+      if (c) ix.release(std::move(c));
+    }
+  } release_ctx{ix, ctx};
+  hipStream_t s = (dev_io && x->stream) ? static_cast<hipStream_t>(x->stream) : (ctx ? ctx->stream : nullptr);
+  const bool timing = ctx && ix.timing.load();
   const katoann_nprobe_model *model = x ? x->nprobe_model : nullptr;
   const bool coarse_only = x && x->coarse_only;
-  if (model) {
-    c.npq.ensure(z.npq);
-    c.mx16.ensure(z.mx16);
-    c.mh1.ensure(z.mh);
-    c.mh2.ensure(z.mh);
-    c.mz.ensure(z.q4);
-  }
-  const uint32_t *npq = nullptr;
-  if (x && x->nprobe_per_query) {
-    c.npq.ensure(z.npq);
-    HIPCHECK(hipMemcpyAsync(c.npq.p, x->nprobe_per_query, B * 4ull,
-                            dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-    npq = c.npq.as<uint32_t>();
-  }
-  if (model) npq = c.npq.as<uint32_t>();  // written by the model's kernels (also on graph replays)
-  auto mark = [&](int i) {
-    if (timing) HIPCHECK(hipEventRecord(c.ev[i], s));
-  };
+  const bool caller_npq = x && x->nprobe_per_query;
+  const uint32_t shift = z.shift, qc = z.qc, nslots = z.nslots, nbands = z.nbands, nstate = z.nstate, rt = z.rt;
+  const uint64_t max_rows = z.max_rows;
+  const uint32_t item_cap = z.item_cap;
   // ranking blocks: as many as are resident at once
   const bool full = z.full;
   const uint32_t grid = static_cast<uint32_t>(device_cus()) *
@@ -770,53 +825,51 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                          : full              ? katoann_hip::rank_full_blocks_per_cu()
                          : z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant)
                                              : 2u);
-  uint32_t *dids = dev_io ? out_ids : c.ids.as<uint32_t>();
-  float *ddst = dev_io ? out_dists : c.dists.as<float>();
-  const float *yq = c.q.as<float>();
+  Workspace w{};
+  const uint32_t *npq = nullptr;
+  uint32_t *dids = nullptr;
+  float *ddst = nullptr;
+  const float *yq = nullptr;
   uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+  auto mark = [&](int i) {
+    if (timing) HIPCHECK(hipEventRecord(ctx->ev[i], s));
+  };
 
-  // pool: S slots plus one band's kept keys per query; on overflow the batch
-  // runs again with room for every candidate of the query
   // the kernels of one pass (normalisation through top-k), launched eagerly
   // or captured once into a graph and replayed while every launch argument
   // stays the same
   auto launches = [&](uint32_t stride) {
     mark(0);
-    if (x && x->normalize) katoann_hip::launch_normalize(c.q.as<float>(), B, d, s);
-    yq = c.q.as<float>();
+    if (x && x->normalize) katoann_hip::launch_normalize(w.q, B, d, s);
+    yq = w.q;
     if (ix.rotated) {
-      katoann_hip::launch_rotate(c.q.as<float>(), B, d, ix.rot.as<float>(), c.y.as<float>(), s);
-      yq = c.y.as<float>();
+      katoann_hip::launch_rotate(w.q, B, d, ix.rot.as<float>(), w.y, s);
+      yq = w.y;
     }
     if (z.bf16)  // src/index.cpp:1230-1237
-      katoann_hip::launch_to_bf16(c.q.as<float>(), B, d, ix.dp, c.qbf.as<uint16_t>(), c.qbn.as<float>(), s);
+      katoann_hip::launch_to_bf16(w.q, B, d, ix.dp, w.qbf, w.qbn, s);
     else
-      katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, c.qhi.as<int8_t>(),
-                                   c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), c.qhn.as<int32_t>(),
-                                   c.qh0.as<int32_t>(), prefix ? c.qpre.as<int8_t>() : nullptr,
-                                   (x && x->codes) ? c.codes.as<uint16_t>() : nullptr, s);
+      katoann_hip::launch_quantize(yq, B, d, dc, ix.head_dims, ix.lo, ix.inv_step, w.qhi, w.qlo, w.qnorm, w.qhn,
+                                   w.qh0, prefix ? w.qpre : nullptr, w.codes, s);
     mark(1);
     if (ix.batched)
-      katoann_hip::launch_coarse(c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), c.qnorm.as<int64_t>(), B,
-                                 ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(), ix.cen_norm.as<int64_t>(), nlist,
-                                 dc, ix.step_sq, c.dist.as<float>(), s);
+      katoann_hip::launch_coarse(w.qhi, w.qlo, w.qnorm, B, ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(),
+                                 ix.cen_norm.as<int64_t>(), nlist, dc, ix.step_sq, w.dist, s);
     else
-      katoann_hip::launch_coarse_flat(c.q.as<float>(), B, d, ix.cen_f.as<float>(), nlist, c.dist.as<float>(), s);
+      katoann_hip::launch_coarse_flat(w.q, B, d, ix.cen_f.as<float>(), nlist, w.dist, s);
     mark(2);
-    katoann_hip::launch_select(c.dist.as<float>(), B, nlist, np, c.probes.as<uint32_t>(), c.probe_d.as<float>(),
-                               s);
+    katoann_hip::launch_select(w.dist, B, nlist, np, w.probes, w.probe_d, s);
     if (model) {  // per-query nprobe from the sorted coarse distances
       const uint32_t F = model->nfeat, H = model->hidden;
-      katoann_hip::launch_features(c.probe_d.as<float>(), B, np, F, model->feat_mean, model->feat_inv_std, nullptr,
-                                   c.mx16.p, s);
-      katoann_hip::launch_gemm_bf16(B, H, F, c.mx16.p, F, 1, model->w1, F, 1, model->b1, nullptr, 0, 1, nullptr,
-                                    c.mh1.p, H, s);
-      katoann_hip::launch_gemm_bf16(B, H, H, c.mh1.p, H, 1, model->w2, H, 1, model->b2, nullptr, 0, 1, nullptr,
-                                    c.mh2.p, H, s);
-      katoann_hip::launch_gemm_bf16(B, 1, H, c.mh2.p, H, 1, model->w3, H, 1, model->b3, nullptr, 0, 0,
-                                    c.mz.as<float>(), nullptr, 1, s);
-      katoann_hip::launch_nprobe_of(c.mz.as<float>(), B, model->y_mean, model->y_std, model->margin,
-                                    std::max(1u, std::min(model->nmin, np)), np, c.npq.as<uint32_t>(), s);
+      katoann_hip::launch_features(w.probe_d, B, np, F, model->feat_mean, model->feat_inv_std, nullptr, w.mx16, s);
+      katoann_hip::launch_gemm_bf16(B, H, F, w.mx16, F, 1, model->w1, F, 1, model->b1, nullptr, 0, 1, nullptr, w.mh1,
+                                    H, s);
+      katoann_hip::launch_gemm_bf16(B, H, H, w.mh1, H, 1, model->w2, H, 1, model->b2, nullptr, 0, 1, nullptr, w.mh2,
+                                    H, s);
+      katoann_hip::launch_gemm_bf16(B, 1, H, w.mh2, H, 1, model->w3, H, 1, model->b3, nullptr, 0, 0, w.mz, nullptr, 1,
+                                    s);
+      katoann_hip::launch_nprobe_of(w.mz, B, model->y_mean, model->y_std, model->margin,
+                                    std::max(1u, std::min(model->nmin, np)), np, w.npq, s);
     }
     mark(3);
     if (coarse_only) {
@@ -825,84 +878,82 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       mark(6);
       return;
     }
-    katoann_hip::launch_plan(c.dist.as<float>(), c.probes.as<uint32_t>(), c.probe_d.as<float>(), npq, B, np,
-                             ix.list_off_d.as<uint64_t>(), nlist, qc, rt, c.pscratch.as<uint32_t>(),
-                             c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                             c.qlist.as<uint32_t>(), c.totals.as<uint32_t>(),
-                             c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(), c.thr.as<uint64_t>(),
-                             c.lbound.as<int32_t>(), c.state.as<uint32_t>(), nstate, shift, z.seed, s);
-    katoann_hip::launch_items(c.item_off.as<uint32_t>(), nslots, nlist, ix.list_off_d.as<uint64_t>(),
-                              c.gcnt.as<uint32_t>(), c.qoff.as<uint32_t>(), qc, rt, item_cap, z.seed, c.items.as<uint4>(),
-                              s);
+    katoann_hip::launch_plan(w.dist, w.probes, w.probe_d, npq, caller_npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
+                             qc, rt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals, w.pool_cnt,
+                             w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, s);
+    katoann_hip::launch_items(w.item_off, nslots, nlist, ix.list_off_d.as<uint64_t>(), w.gcnt, w.qoff, qc, rt,
+                              item_cap, z.seed, w.items, s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
-    const int32_t *qn = prefix ? c.qh0.as<int32_t>() : c.qhn.as<int32_t>();
-    const int8_t *qrank = prefix ? c.qpre.as<int8_t>() : c.qhi.as<int8_t>();
+    const int32_t *qn = prefix ? w.qh0 : w.qhn;
+    const int8_t *qrank = prefix ? w.qpre : w.qhi;
     const uint32_t qstride = prefix ? P : dc;
     for (uint32_t band = 0; band < nbands; ++band) {
-      if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band], s));
+      if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band], s));
       if (z.bf16)
         katoann_hip::launch_rank_bf16(band, ix.bf.as<uint16_t>(), ix.bfn.as<float>(), ix.ids.as<uint32_t>(), nlist,
-                                      c.qbf.as<uint16_t>(), c.qbn.as<float>(), ix.dp, c.thr.as<uint64_t>(),
-                                      c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride,
-                                      c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
+                                      w.qbf, w.qbn, ix.dp, w.thr, w.qlist, w.item_off, w.items, stride, w.pool,
+                                      w.pool_cnt, w.state, grid, s);
       else if (full)
         katoann_hip::launch_rank_full(band, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
-                                      ix.ids.as<uint32_t>(), nlist, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(), dc,
-                                      c.qnorm.as<int64_t>(), c.thr.as<uint64_t>(), c.qlist.as<uint32_t>(),
-                                      c.item_off.as<uint32_t>(), c.items.as<uint4>(), stride, ix.step_sq,
-                                      c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
+                                      ix.ids.as<uint32_t>(), nlist, w.qhi, w.qlo, dc, w.qnorm, w.thr, w.qlist,
+                                      w.item_off, w.items, stride, ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s);
       const bool launched =
-          full ? true
-          : z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
-                                                 qrank, qstride, qn, c.lbound.as<int32_t>(),
-                                                 c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                                                 c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
-                                                 c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s)
-                 : katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P,
-                                                 qrank, qstride, qn, c.lbound.as<int32_t>(),
-                                                 c.qlist.as<uint32_t>(), c.item_off.as<uint32_t>(),
-                                                 c.items.as<uint4>(), stride, ix.step_sq, c.pool.as<uint64_t>(),
-                                                 c.pool_cnt.as<uint32_t>(), c.state.as<uint32_t>(), grid, s);
+          full     ? true
+          : z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist,
+                                                   P, qrank, qstride, qn, w.lbound, w.qlist, w.item_off, w.items,
+                                                   stride, ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
+                   : katoann_hip::launch_rank_band(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank, qstride,
+                                                   qn, w.lbound, w.qlist, w.item_off, w.items, stride, ix.step_sq,
+                                                   w.pool, w.pool_cnt, w.state, grid, s);
       if (!launched) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
-      if (timing) HIPCHECK(hipEventRecord(c.bev[2 * band + 1], s));
-      if (!katoann_hip::launch_update(c.pool.as<uint64_t>(), c.pool_cnt.as<uint32_t>(), c.pool_sorted.as<uint32_t>(),
-                                      c.thr.as<uint64_t>(), c.lbound.as<int32_t>(), B, S, stride, P, ix.step_sq, s))
+      if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band + 1], s));
+      if (!katoann_hip::launch_update(w.pool, w.pool_cnt, w.pool_sorted, w.thr, w.lbound, B, S, stride, P, ix.step_sq,
+                                      s))
         throw std::invalid_argument("unsupported survivor count");
     }
     mark(5);
     if (z.bf16)
-      katoann_hip::launch_emit_bf16(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), B, K, dids, ddst,
-                                    (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
-                                    (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s);
-    else if (!katoann_hip::launch_refine(c.pool.as<uint64_t>(), stride, c.pool_cnt.as<uint32_t>(), c.thr.as<uint64_t>(),
-                                    B, S, K, ix.hi.as<int8_t>(), ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(),
-                                    ix.id_row.as<uint32_t>(), ix.npts, dc, c.qhi.as<int8_t>(), c.qlo.as<int8_t>(),
-                                    c.qnorm.as<int64_t>(), ix.step_sq, dids, ddst,
-                                    (x && x->survivors) ? c.surv.as<uint64_t>() : nullptr,
-                                    (x && x->n_survivors) ? c.nsurv.as<uint32_t>() : nullptr, s))
+      katoann_hip::launch_emit_bf16(w.pool, stride, w.pool_cnt, B, K, dids, ddst, w.surv, w.nsurv, s);
+    else if (!katoann_hip::launch_refine(w.pool, stride, w.pool_cnt, w.thr, B, S, K, ix.hi.as<int8_t>(),
+                                         ix.lo_plane.as<int8_t>(), ix.norm.as<int64_t>(), ix.id_row.as<uint32_t>(),
+                                         ix.npts, dc, w.qhi, w.qlo, w.qnorm, ix.step_sq, dids, ddst, w.surv, w.nsurv,
+                                         s))
       throw std::invalid_argument("unsupported refinement shape");
     mark(6);
   };
-  const bool graphs = mix.graphs.load() && !timing;  // stage timing runs eagerly, events on the stream
+  const bool graphs = ctx && mix.graphs.load() && !timing;  // stage timing runs eagerly, events on the stream
+  size_t ws_bytes = 0;
   auto run = [&](uint64_t allowance) {
-    const uint32_t stride = static_cast<uint32_t>(std::min<uint64_t>(S + allowance, 0xffffffffu));
-    c.pool.ensure(std::max<uint64_t>(static_cast<uint64_t>(stride) * B, 1) * 8);
-    HIPCHECK(hipMemcpyAsync(c.q.p, queries, static_cast<size_t>(B) * d * 4,
+    const uint32_t stride = static_cast<uint32_t>(pool_stride(z, allowance));
+    ws_bytes = carve(z, stride, nullptr, nullptr);
+    char *base = nullptr;
+    if (fixed) {
+      if (ws_bytes > fixed->bytes) throw std::logic_error("unreachable: workspace above the fixed scratch");
+      base = fixed->base;
+    } else {
+      ctx->ws.ensure(ws_bytes);
+      base = ctx->ws.as<char>();
+    }
+    carve(z, stride, base, &w);
+    npq = (model || caller_npq) ? w.npq : nullptr;
+    dids = dev_io ? out_ids : w.ids;
+    ddst = dev_io ? out_dists : w.dists;
+    if (caller_npq)
+      HIPCHECK(hipMemcpyAsync(w.npq, x->nprobe_per_query, B * 4ull,
+                              dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(w.q, queries, static_cast<size_t>(B) * d * 4,
                             dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     if (graphs) {
-      // everything a launch reads: shapes, scalars, every buffer, the stream
+      SearchCtx &c = *ctx;
+      // everything a launch reads: shapes, scalars, the workspace, the stream
       std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
                                    rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, full, z.bf16, grid, ix.gen,
-                                   ix.rotated, x && x->normalize, coarse_only, reinterpret_cast<uint64_t>(npq),
+                                   ix.rotated, x && x->normalize, coarse_only, caller_npq, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
-                                   x && x->survivors, x && x->n_survivors, x && x->codes};
-      for (const DevBuf *b : {&c.q, &c.y, &c.qhi, &c.qlo, &c.qpre, &c.qbf, &c.qbn, &c.qnorm, &c.qhn, &c.qh0, &c.dist, &c.probes, &c.probe_d,
-                              &c.npq, &c.gcnt, &c.qoff, &c.pscratch, &c.item_off, &c.qlist, &c.totals, &c.pool_cnt,
-                              &c.pool_sorted, &c.thr, &c.lbound, &c.state, &c.pool, &c.surv, &c.nsurv, &c.codes,
-                              &c.items, &c.mx16, &c.mh1, &c.mh2, &c.mz})
-        key.push_back(reinterpret_cast<uint64_t>(b->p));
+                                   reinterpret_cast<uint64_t>(base), ws_bytes, z.surv, z.nsurv, z.codes, z.qpre,
+                                   z.qbf, z.npq, z.mx16, z.mh};
       if (model) {
         uint64_t words[(sizeof(katoann_nprobe_model) + 7) / 8] = {};
         std::memcpy(words, model, sizeof(katoann_nprobe_model));
@@ -935,34 +986,49 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       HIPCHECK(hipStreamSynchronize(s));
       return true;
     }
-    HIPCHECK(hipMemcpyAsync(st, c.state.p, 12, hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(st + 4, c.totals.p, 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(st, w.state, 12, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(st + 4, w.totals, 8, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    if (st[2]) throw std::invalid_argument("per-query nprobe must be in [1, nprobe]");
     return st[0] == 0;
   };
+  // Pool room: the plan's; within fixed scratch, at most what fits there.
+  uint64_t allowance = z.allowance;
+  uint64_t room_cap = max_rows;
+  if (fixed) {
+    if (carve(z, pool_stride(z, 0), nullptr, nullptr) > fixed->bytes)
+      throw std::invalid_argument("search needs " + std::to_string(carve(z, pool_stride(z, 0), nullptr, nullptr)) +
+                                  " bytes of scratch, above the allocation-free search's " +
+                                  std::to_string(fixed->bytes));
+    const size_t rest = carve(z, 0, nullptr, nullptr) - 256;  // everything but the pool (its one 256-byte slot)
+    const uint64_t fit = (fixed->bytes - rest) / 8 / B;
+    room_cap = std::min<uint64_t>(room_cap, fit > S ? fit - S : 0);
+    allowance = std::min(allowance, room_cap);
+  }
+  const uint64_t limit = (x && x->max_bytes) ? x->max_bytes : 0;
   // on a pool overflow the batch runs again with 4x the room, up to every
   // candidate; later searches of the shape start from the room that sufficed
-  const std::array<uint64_t, 8> akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
-                                        uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) |
-                                            (uint64_t(z.full) << 10) | (uint64_t(z.bf16) << 11)};
-  uint64_t allowance = z.allowance;
-  {
-    std::lock_guard<std::mutex> g(mix.tune_mu);
-    auto it = mix.allowance_seen.find(akey);
-    if (it != mix.allowance_seen.end()) allowance = std::max(allowance, it->second);
-  }
-  const uint64_t first = allowance;
   while (!run(allowance)) {
-    if (allowance >= max_rows) {
-      ix.release(std::move(ctx));
+    if (allowance >= room_cap) {
+      if (fixed && room_cap < max_rows)
+        throw std::runtime_error("survivor pool overflow in the ranking pass: the batch needs more room than the "
+                                 "allocation-free search's scratch holds");
       throw std::runtime_error("survivor pool overflow in the ranking pass");
     }
-    allowance = std::min<uint64_t>(max_rows, allowance * 4);
+    const uint64_t next = std::min<uint64_t>(room_cap, allowance * 4);
+    const size_t need = carve(z, pool_stride(z, next), nullptr, nullptr);
+    if (!fixed) {
+      std::lock_guard<std::mutex> g(mix.tune_mu);
+      mix.allowance_seen[z.akey] = std::max(mix.allowance_seen[z.akey], next);
+    }
+    if (limit && need > limit) {
+      if (x->needed_bytes) *x->needed_bytes = need;
+      throw std::runtime_error("survivor pool overflow: the rerun needs " + std::to_string(need) +
+                               " bytes of search buffers, above the " + std::to_string(limit) + " allowed");
+    }
+    allowance = next;
   }
-  if (allowance != first) {
-    std::lock_guard<std::mutex> g(mix.tune_mu);
-    mix.allowance_seen[akey] = allowance;
-  }
+  if (x && x->needed_bytes) *x->needed_bytes = ws_bytes;
 
   const hipMemcpyKind d2o = dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (!dev_io && !coarse_only) {
@@ -970,29 +1036,28 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     HIPCHECK(hipMemcpyAsync(out_dists, ddst, static_cast<size_t>(B) * K * 4, hipMemcpyDeviceToHost, s));
   }
   if (x) {
-    if (x->probes) HIPCHECK(hipMemcpyAsync(x->probes, c.probes.p, static_cast<size_t>(B) * np * 4, d2o, s));
-    if (x->probe_dists)
-      HIPCHECK(hipMemcpyAsync(x->probe_dists, c.probe_d.p, static_cast<size_t>(B) * np * 4, d2o, s));
-    if (x->survivors) HIPCHECK(hipMemcpyAsync(x->survivors, c.surv.p, static_cast<size_t>(B) * S * 8, d2o, s));
-    if (x->n_survivors) HIPCHECK(hipMemcpyAsync(x->n_survivors, c.nsurv.p, B * 4ull, d2o, s));
+    if (x->probes) HIPCHECK(hipMemcpyAsync(x->probes, w.probes, static_cast<size_t>(B) * np * 4, d2o, s));
+    if (x->probe_dists) HIPCHECK(hipMemcpyAsync(x->probe_dists, w.probe_d, static_cast<size_t>(B) * np * 4, d2o, s));
+    if (x->survivors) HIPCHECK(hipMemcpyAsync(x->survivors, w.surv, static_cast<size_t>(B) * S * 8, d2o, s));
+    if (x->n_survivors) HIPCHECK(hipMemcpyAsync(x->n_survivors, w.nsurv, B * 4ull, d2o, s));
     if (x->rotated) HIPCHECK(hipMemcpyAsync(x->rotated, yq, static_cast<size_t>(B) * d * 4, d2o, s));
-    if (x->codes) HIPCHECK(hipMemcpyAsync(x->codes, c.codes.p, static_cast<size_t>(B) * dc * 2, d2o, s));
+    if (x->codes) HIPCHECK(hipMemcpyAsync(x->codes, w.codes, static_cast<size_t>(B) * dc * 2, d2o, s));
     if (x->nprobe_used) {
       if (npq) {
         HIPCHECK(hipMemcpyAsync(x->nprobe_used, npq, B * 4ull, d2o, s));
+      } else if (dev_io) {
+        katoann_hip::launch_fill32(x->nprobe_used, B, np, s);
+        HIPCHECK(hipGetLastError());
       } else {
         HIPCHECK(hipStreamSynchronize(s));
-        std::vector<uint32_t> v(B, np);
-        HIPCHECK(hipMemcpyAsync(x->nprobe_used, v.data(), B * 4ull,
-                                dev_io ? hipMemcpyHostToDevice : hipMemcpyHostToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        std::fill(x->nprobe_used, x->nprobe_used + B, np);
       }
     }
   }
   HIPCHECK(hipStreamSynchronize(s));
   if (x && npq && !dev_io && (x->probes || x->probe_dists)) {  // past a query's own count: no list
     std::vector<uint32_t> hn(B);
-    if (x->nprobe_per_query) std::copy(x->nprobe_per_query, x->nprobe_per_query + B, hn.begin());
+    if (caller_npq) std::copy(x->nprobe_per_query, x->nprobe_per_query + B, hn.begin());
     else HIPCHECK(hipMemcpy(hn.data(), npq, B * 4ull, hipMemcpyDeviceToHost));
     for (uint32_t q = 0; q < B; ++q)
       for (uint32_t j = hn[q]; j < np; ++j) {
@@ -1004,11 +1069,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     std::lock_guard<std::mutex> g(mix.stat_mu);
     if (timing) {
       float ms[6] = {};
-      for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]);
+      for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
       float rank = 0.0f;
       for (uint32_t b = 0; b < nbands; ++b) {
         float t = 0.0f;
-        (void)hipEventElapsedTime(&t, c.bev[2 * b], c.bev[2 * b + 1]);
+        (void)hipEventElapsedTime(&t, ctx->bev[2 * b], ctx->bev[2 * b + 1]);
         rank += t;
       }
       const float out[KATOANN_NSTAGES] = {ms[0], ms[1], ms[2], ms[3], rank, std::max(0.0f, ms[4] - rank), ms[5]};
@@ -1019,7 +1084,6 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     mix.work[2] = st[1];  // candidates kept
     mix.work[3] = nbands;
   }
-  ix.release(std::move(ctx));
 }
 
 // Row sums of shifted bytes for the kernel-level API's exact unsigned products.
@@ -1191,6 +1255,20 @@ int katoann_search(const katoann_index *h, const float *queries, uint32_t nq, co
   KATOANN_CATCH
 }
 
+// katoann_search_ex (device pointers) in `bytes` of caller-provided device
+// memory instead of buffers of its own: no autotuning, graphs or stage timing,
+// on x->stream; the caller keeps calls that share the memory apart.
+int katoann_search_in_scratch(const katoann_index *h, const float *queries, uint32_t nq,
+                              const katoann_search_params *p, const katoann_search_extras *x, uint32_t *ids,
+                              float *dists, void *scratch, uint64_t bytes) {
+  KATOANN_TRY
+  if (!h || !p || !x) throw std::invalid_argument("null argument");
+  if (!x->device_io) throw std::invalid_argument("the allocation-free search takes device pointers (device_io)");
+  const FixedScratch fs{static_cast<char *>(scratch), bytes};
+  search_impl(*h, queries, nq, *p, x, ids, dists, nullptr, &fs);
+  KATOANN_CATCH
+}
+
 int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
                       const katoann_search_extras *x, uint32_t *ids, float *dists) {
   KATOANN_TRY
@@ -1223,7 +1301,12 @@ int katoann_search_footprint(const katoann_index *h, uint32_t nq, const katoann_
                              const katoann_search_extras *x, uint64_t *bytes) {
   KATOANN_TRY
   if (!h || !p || !bytes) throw std::invalid_argument("null argument");
-  *bytes = nq ? plan_search(*h, nq, *p, x).total() : 0;
+  if (nq) {
+    const SearchPlan z = plan_search(*h, nq, *p, x);
+    *bytes = carve(z, pool_stride(z, z.allowance), nullptr, nullptr);
+  } else {
+    *bytes = 0;
+  }
   KATOANN_CATCH
 }
 

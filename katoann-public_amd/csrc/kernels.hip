@@ -810,6 +810,19 @@ __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Per-query probe counts from the caller: a count outside [1, np] sets
+// state[2] (the host turns it into std::invalid_argument) and is clamped
+// into range, so the kernels after it read only probed ranks.
+__global__ void k_check_npq(uint32_t *npq, uint32_t B, uint32_t np, uint32_t *state) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= B) return;
+  const uint32_t v = npq[q];
+  if (v == 0 || v > np) {
+    atomicOr(&state[2], 1u);
+    npq[q] = v == 0 ? 1u : np;
+  }
+}
+
 constexpr int INV_QT = 64;    // queries per tile
 constexpr int INV_MAXB = 16;  // bands at most (nprobe 4096, F = 2: 13)
 
@@ -1291,11 +1304,15 @@ static void inv_pass_launch(uint32_t nb, dim3 tiles, hipStream_t s, const float 
   }
 }
 
-void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
-                 uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
-                 uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
-                 uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
-                 uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
+void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+
+void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, bool check_npq,
+                 uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
+                 uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist,
+                 uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
+                 uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
   const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
   const bool from_probes = inv_from_probes(nlist, np);
   const uint32_t qtiles = from_probes ? 1u : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
@@ -1303,6 +1320,8 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   uint32_t *seed_r0 = bsum + 2 * nblk;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
   hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
+  if (npq && check_npq)
+    hipLaunchKernelGGL(k_check_npq, dim3((B + 255) / 256), dim3(256), 0, s, const_cast<uint32_t *>(npq), B, np, state);
   if (seed)
     hipLaunchKernelGGL(k_seed_ranks, dim3((B + 255) / 256), dim3(256), 0, s, probes, npq, list_off, B, np, seed,
                        seed_r0);

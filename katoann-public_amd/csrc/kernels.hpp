@@ -61,12 +61,15 @@ inline bool inv_from_probes(uint32_t nlist, uint32_t np) { return 16ull * np <= 
 // 32-bit words of plan scratch for a batch (tile partial counts, item counts,
 // block sums, seed ranks).
 size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed = 0);
-// Inversion and work plan from the coarse matrix and the sorted probes.
-void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, uint32_t B,
-                 uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt, uint32_t *scratch,
-                 uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist, uint32_t *totals,
-                 uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound, uint32_t *state,
-                 uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
+// Inversion and work plan from the coarse matrix and the sorted probes. With
+// check_npq, per-query counts outside [1, np] are clamped in place and flag
+// state[2].
+void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, bool check_npq,
+                 uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
+                 uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist,
+                 uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
+                 uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
+void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,

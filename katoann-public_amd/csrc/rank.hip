@@ -1372,6 +1372,63 @@ struct RefineArgs {
 };
 
 
+// Exact keys (f(l2) bits << 32 | id) of 16 survivors: lane i < 16 returns
+// survivor i's, where every lane passes the key of survivor lane % 16 and
+// whether it exists. The survivors' rows are gathered by id (A = 16 rows of
+// one plane), the query's high and low bytes are the B operand's columns 0 and
+// 1; the four byte products are combined in int64. A survivor whose id has no
+// row gets kNoKey.
+__device__ inline uint64_t exact_key16(const RefineArgs &a, uint64_t key, bool sok, const int8_t *qh, const int8_t *ql,
+                                       int64_t qn) {
+  const int lane = lane_id();
+  const int koff = 16 * (lane >> 4), col = lane & 15;
+  const uint32_t id = static_cast<uint32_t>(key);
+  sok = sok && id < a.npts;
+  const uint32_t row = sok ? a.id_row[id] : 0xffffffffu;
+  const bool rok = sok && row != 0xffffffffu;
+  v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+  // RB chunks of 64 bytes in flight at once: the rows are random gathers
+  constexpr int RB = 16;
+  for (uint32_t kc0 = 0; kc0 < a.dc; kc0 += 64 * RB) {
+    v4i xh[RB], xl[RB], bq[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const uint32_t kc = kc0 + 64 * u;
+      const bool in = kc < a.dc;
+      xh[u] = rok && in ? load16(a.hi + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
+      xl[u] = rok && in ? load16(a.lo + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
+      bq[u] = in && col == 0 ? load16(qh + kc + koff) : (in && col == 1 ? load16(ql + kc + koff) : v4i{0, 0, 0, 0});
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      ah = mfma(xh[u], bq[u], ah);
+      al = mfma(xl[u], bq[u], al);
+    }
+  }
+  // survivor i = lane (< 16): C[i][0] in lane 16 (i/4), element i % 4; C[i][1] one lane up
+  const int i = lane & 15;
+  const int src = 16 * (i >> 2);
+  int32_t hh = 0, hl = 0, lh = 0, ll = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int32_t a0 = __shfl(ah[r], src), a1 = __shfl(ah[r], src + 1);
+    const int32_t b0 = __shfl(al[r], src), b1 = __shfl(al[r], src + 1);
+    if ((i & 3) == r) {
+      hh = a0;
+      hl = a1;
+      lh = b0;
+      ll = b1;
+    }
+  }
+  const bool mine_ok = __shfl(static_cast<int>(rok), i) != 0;
+  const uint32_t myrow = __shfl(row, i);
+  const uint32_t myid = __shfl(id, i);
+  if (!mine_ok) return kNoKey;
+  const int64_t dot =
+      static_cast<int64_t>(hh) * 65536 + (static_cast<int64_t>(hl) + lh) * 256 + static_cast<int64_t>(ll);
+  return make_key(fscale(qn + a.norm[myrow] - 2 * dot, a.step_sq), myid);
+}
+
 // Final survivors, exact distances, arrival rule, output (src/index.cpp:1046-1127):
 //  1. the S smallest keys of the query's pool (exactly the S smallest ranking
 //     keys of all its candidates);
@@ -1427,62 +1484,11 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
   const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
   const int64_t qn = a.qnorm[q];
-  const int koff = 16 * (lane >> 4), col = lane & 15;
   for (uint32_t t0 = 0; t0 < ns; t0 += 16) {
     const uint32_t s = t0 + (lane & 15);
     const uint64_t key = top_get<SC>(sv, min(s, static_cast<uint32_t>(SC - 1)));
-    const uint32_t id = static_cast<uint32_t>(key);
-    const bool sok = s < ns && id < a.npts;
-    const uint32_t row = sok ? a.id_row[id] : 0xffffffffu;
-    const bool rok = sok && row != 0xffffffffu;
-    v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
-    // RB chunks of 64 bytes in flight at once: the rows are random gathers
-    constexpr int RB = 16;
-    for (uint32_t kc0 = 0; kc0 < a.dc; kc0 += 64 * RB) {
-      v4i xh[RB], xl[RB], bq[RB];
-#pragma unroll
-      for (int u = 0; u < RB; ++u) {
-        const uint32_t kc = kc0 + 64 * u;
-        const bool in = kc < a.dc;
-        xh[u] = rok && in ? load16(a.hi + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-        xl[u] = rok && in ? load16(a.lo + static_cast<size_t>(row) * a.dc + kc + koff) : v4i{0, 0, 0, 0};
-        bq[u] = in && col == 0 ? load16(qh + kc + koff)
-                               : (in && col == 1 ? load16(ql + kc + koff) : v4i{0, 0, 0, 0});
-      }
-#pragma unroll
-      for (int u = 0; u < RB; ++u) {
-        ah = mfma(xh[u], bq[u], ah);
-        al = mfma(xl[u], bq[u], al);
-      }
-    }
-    // survivor i = lane (< 16): C[i][0] in lane 16 (i/4), element i % 4; C[i][1] one lane up
-    const int i = lane & 15;
-    const int src = 16 * (i >> 2);
-    int32_t hh = 0, hl = 0, lh = 0, ll = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int32_t a0 = __shfl(ah[r], src), a1 = __shfl(ah[r], src + 1);
-      const int32_t b0 = __shfl(al[r], src), b1 = __shfl(al[r], src + 1);
-      if ((i & 3) == r) {
-        hh = a0;
-        hl = a1;
-        lh = b0;
-        ll = b1;
-      }
-    }
-    const uint32_t si = t0 + i;
-    const bool mine_ok = __shfl(static_cast<int>(rok), i) != 0;
-    const uint32_t myrow = __shfl(row, i);
-    const uint32_t myid = __shfl(id, i);
-    if (lane < 16 && si < ns) {
-      uint64_t e = kNoKey;
-      if (mine_ok) {
-        const int64_t dot = static_cast<int64_t>(hh) * 65536 + (static_cast<int64_t>(hl) + lh) * 256 +
-                            static_cast<int64_t>(ll);
-        e = make_key(fscale(qn + a.norm[myrow] - 2 * dot, a.step_sq), myid);
-      }
-      ex[w][si] = e;
-    }
+    const uint64_t e = exact_key16(a, key, s < ns, qh, ql, qn);
+    if (lane < 16 && t0 + lane < ns) ex[w][t0 + lane] = e;
   }
   wave_fence();
 
@@ -1534,12 +1540,178 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   }
 }
 
+// ---- survivor counts above 1,024 (up to BIG_SB): one block per query, lists in LDS
+constexpr uint32_t BIG_SB = 4096;
+constexpr size_t BIG_LDS = 3 * BIG_SB * sizeof(uint64_t);  // one list and the sort's two buffers
+
+// Ascending sort of a bitonic sequence a[0, M) in LDS (256 threads).
+__device__ inline void block_bitonic_merge(uint64_t *a, uint32_t M) {
+  for (uint32_t j = M >> 1; j > 0; j >>= 1) {
+    for (uint32_t p = threadIdx.x; p < M / 2; p += 256) {
+      const uint32_t i = (p / j) * 2 * j + p % j;
+      const uint64_t x = a[i], y = a[i + j];
+      a[i] = umin64(x, y);
+      a[i + j] = umax64(x, y);
+    }
+    __syncthreads();
+  }
+}
+
+// Keys below kNoKey in the ascending list a[0, n): n minus the first kNoKey's index.
+__device__ inline uint32_t block_count_sorted(const uint64_t *a, uint32_t n, uint32_t *tmp) {
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) c += a[i] != kNoKey ? 1u : 0u;
+  const uint32_t off = block_exscan256(c, tmp);
+  __shared__ uint32_t total;
+  if (threadIdx.x == 255) total = off + c;
+  __syncthreads();
+  const uint32_t t = total;
+  __syncthreads();
+  return t;
+}
+
+// k_update for S > 1024: the previous sorted list, then the new keys in
+// chunks of BIG_SB, each chunk sorted and merged in (the BIG_SB smallest of
+// list and chunk by one bitonic merge, entries past S dropped); keys at or
+// above the list's S-th are dropped before sorting.
+__global__ __launch_bounds__(256) void k_update_big(UpdateArgs a) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t stmp[4];
+  uint64_t *cur = lds, *buf = lds + BIG_SB;
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride);
+  const uint32_t sorted = a.pool_sorted[q] & 0x7fffffffu;
+  if (n == sorted) return;
+  uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
+  for (uint32_t i = tid; i < BIG_SB; i += 256) cur[i] = i < sorted ? pq[i] : kNoKey;
+  __syncthreads();
+  uint64_t cap = cur[a.S - 1];  // kNoKey until the list holds S keys
+  for (uint32_t b0 = sorted; b0 < n; b0 += BIG_SB) {
+    for (uint32_t i = tid; i < BIG_SB; i += 256) {
+      const uint64_t k = b0 + i < n ? pq[b0 + i] : kNoKey;
+      buf[i] = k < cap ? k : kNoKey;
+    }
+    __syncthreads();
+    const uint64_t *srt = block_sort_keys(buf, BIG_SB);
+    // cur ascending, srt ascending: min(cur[i], srt[M - 1 - i]) is a bitonic
+    // sequence holding the BIG_SB smallest of both
+    for (uint32_t i = tid; i < BIG_SB; i += 256) cur[i] = umin64(cur[i], srt[BIG_SB - 1 - i]);
+    __syncthreads();
+    block_bitonic_merge(cur, BIG_SB);
+    for (uint32_t i = a.S + tid; i < BIG_SB; i += 256) cur[i] = kNoKey;
+    __syncthreads();
+    cap = cur[a.S - 1];
+    __syncthreads();
+  }
+  const uint32_t cnt = block_count_sorted(cur, a.S, stmp);
+  for (uint32_t i = tid; i < cnt; i += 256) pq[i] = cur[i];
+  if (tid == 0) {
+    a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride] = cnt;
+    a.pool_sorted[q] = cnt;
+    if (cnt == a.S) {
+      const uint64_t kth = cur[a.S - 1];
+      if (kth < a.thr[q]) {
+        a.thr[q] = kth;
+        a.lbound[q] = static_cast<int32_t>(l2_bound(key_float(kth), a.step_sq, a.lmax));
+      }
+    }
+  }
+}
+
+// k_refine for S > 1024: the pool after the last update holds the query's
+// survivors sorted ascending; the four waves take groups of 16 for the exact
+// distances; then the arrival rule in closed form (as in k_refine) over the
+// block: survivor i arrives after every survivor above it (i descending).
+__global__ __launch_bounds__(256) void k_refine_big(RefineArgs a) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t stmp[4];
+  __shared__ uint32_t s_mb;
+  uint64_t *ex = lds, *buf = lds + BIG_SB;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
+  const uint32_t m = min(min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride), a.S);
+  const uint64_t tq = a.thr[q];
+  for (uint32_t i = tid; i < BIG_SB; i += 256) {
+    const uint64_t k = i < m ? pq[i] : kNoKey;
+    buf[i] = k <= tq ? k : kNoKey;
+  }
+  __syncthreads();
+  const uint32_t ns = block_count_sorted(buf, BIG_SB, stmp);
+  if (a.surv)
+    for (uint32_t i = tid; i < a.S; i += 256) a.surv[static_cast<size_t>(q) * a.S + i] = buf[i];
+  if (a.nsurv && tid == 0) a.nsurv[q] = ns;
+  const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
+  const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
+  const int64_t qn = a.qnorm[q];
+  for (uint32_t t0 = 16 * w; t0 < ns; t0 += 64) {
+    const uint32_t s = t0 + (lane & 15);
+    const uint64_t key = buf[min(s, BIG_SB - 1)];
+    const uint64_t e = exact_key16(a, key, s < ns, qh, ql, qn);
+    if (lane < 16 && t0 + lane < ns) ex[t0 + lane] = e;
+  }
+  __syncthreads();
+  uint32_t M = 64;
+  while (M < ns) M <<= 1;
+  for (uint32_t i = tid; i < M; i += 256) buf[i] = i < ns ? ex[i] : kNoKey;
+  __syncthreads();
+  const uint64_t *srt = block_sort_keys(buf, M);
+  const uint64_t *fin = srt;
+  if (ns > a.K) {
+    if (tid == 0) s_mb = static_cast<uint32_t>(srt[a.K - 1] >> 32);  // bits of M (dists >= 0)
+    __syncthreads();
+    const uint32_t mb = s_mb;
+    // thread t owns survivors [R t, R t + R); pre = arrivals before it (higher
+    // indices) with dist <= M
+    constexpr uint32_t R = BIG_SB / 256;
+    const uint32_t i0 = R * tid;
+    uint32_t c = 0;
+    for (uint32_t i = i0; i < i0 + R && i < ns; ++i) c += static_cast<uint32_t>(ex[i] >> 32) <= mb ? 1u : 0u;
+    const uint32_t before = block_exscan256(c, stmp);  // in lower-index threads
+    __shared__ uint32_t s_tot;
+    if (tid == 255) s_tot = before + c;
+    __syncthreads();
+    uint32_t pre = s_tot - before - c;  // in higher-index threads
+    __syncthreads();
+    for (uint32_t i = i0 + R; i-- > i0;) {
+      if (i >= ns) continue;
+      const uint64_t k = ex[i];
+      const uint32_t fb = static_cast<uint32_t>(k >> 32);
+      const bool in = k != kNoKey && (fb < mb || (fb == mb && pre < a.K));
+      if (k != kNoKey && fb <= mb) ++pre;
+      ex[i] = in ? k : kNoKey;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < M; i += 256) buf[i] = i < ns ? ex[i] : kNoKey;
+    __syncthreads();
+    fin = block_sort_keys(buf, M);
+  }
+  for (uint32_t g = tid; g < a.K; g += 256) {
+    const uint64_t k = g < M ? fin[g] : kNoKey;
+    const bool ok = k != kNoKey;
+    a.out_ids[static_cast<size_t>(q) * a.K + g] = ok ? static_cast<uint32_t>(k) : 0xffffffffu;
+    a.out_dists[static_cast<size_t>(q) * a.K + g] = ok ? key_float(k) : FLT_MAX;
+  }
+}
+
 }  // namespace
 
 namespace katoann_hip {
 
 uint32_t refine_cap(uint32_t S) {
-  return S <= 64 ? 64 : S <= 128 ? 128 : S <= 256 ? 256 : S <= 512 ? 512 : S <= 1024 ? 1024 : 0;
+  return S <= 64     ? 64
+         : S <= 128  ? 128
+         : S <= 256  ? 256
+         : S <= 512  ? 512
+         : S <= 1024 ? 1024
+         : S <= BIG_SB ? BIG_SB
+                       : 0;
+}
+
+template <class K>
+static void big_lds_attr(K kernel) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(BIG_LDS));
 }
 
 uint32_t rank_queries_per_item(uint32_t P) {
@@ -1718,6 +1890,12 @@ bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, ui
     case 256: hipLaunchKernelGGL(k_update<256>, grid, dim3(256), 0, s, a); return true;
     case 512: hipLaunchKernelGGL(k_update<512>, grid, dim3(256), 0, s, a); return true;
     case 1024: hipLaunchKernelGGL(k_update<1024>, grid, dim3(256), 0, s, a); return true;
+    case BIG_SB: {
+      static const bool attr = (big_lds_attr(&k_update_big), true);
+      (void)attr;
+      hipLaunchKernelGGL(k_update_big, grid, dim3(256), BIG_LDS, s, a);
+      return true;
+    }
     default: return false;
   }
 }
@@ -1737,6 +1915,12 @@ bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_c
     case 256: hipLaunchKernelGGL(k_refine<256>, grid, dim3(256), 0, s, a); return true;
     case 512: hipLaunchKernelGGL(k_refine<512>, grid, dim3(256), 0, s, a); return true;
     case 1024: hipLaunchKernelGGL(k_refine<1024>, grid, dim3(256), 0, s, a); return true;
+    case BIG_SB: {
+      static const bool attr = (big_lds_attr(&k_refine_big), true);
+      (void)attr;
+      hipLaunchKernelGGL(k_refine_big, dim3(B), dim3(256), BIG_LDS, s, a);
+      return true;
+    }
     default: return false;
   }
 }

@@ -10,9 +10,11 @@
 // context of its own (its own HIP stream), so batches of different workers
 // overlap on the GPU. Host staging buffers are pinned (hipHostMalloc) so the
 // copies in and out are asynchronous DMA. The budget bounds the sum of the
-// footprints of the batches in flight; the index keeps an idle search
-// context's buffers for reuse, so the device memory held for searches is at
-// most max_inflight times the largest batch footprint.
+// footprints of the batches in flight: a search may not grow its buffers past
+// its batch's reservation, and a survivor-pool overflow whose rerun needs more
+// room reserves that room first (new batches wait meanwhile). The index keeps
+// an idle search context's buffers for reuse, so the device memory held for
+// searches is at most max_inflight times the largest batch footprint.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -91,7 +93,8 @@ struct katoann_server {
   std::deque<Request *> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_ticket = 1;
-  uint64_t reserved = 0, peak_reserved = 0;
+  uint64_t reserved = 0, peak_reserved = 0, peak_used = 0;
+  uint32_t rerun_waiting = 0;  // batches waiting for room to run again after a pool overflow
   uint32_t inflight = 0, peak_inflight = 0;
   bool stop = false;
   std::vector<std::thread> workers;
@@ -114,20 +117,25 @@ struct katoann_server {
   // Under mu: the next batch that fits, or an empty vector.
   std::vector<Request *> take(uint64_t &bytes) {
     std::vector<Request *> batch;
-    if (queue.empty()) return batch;
-    Request *head = queue.front();
+    if (rerun_waiting) return batch;  // a batch in flight waits for room to run again: it goes first
+    Request *head = nullptr;
     uint64_t need = 0;
-    try {
-      need = footprint(*head, head->nq);
-    } catch (const std::exception &e) {  // checked at submission; fail it rather than the worker
-      head->status = KATOANN_INVALID_ARGUMENT;
-      head->error = e.what();
-      head->done = true;
-      head->done_at = Clock::now();
-      queue.pop_front();
-      cv_done.notify_all();
-      return batch;
+    while (!queue.empty()) {
+      head = queue.front();
+      try {
+        need = footprint(*head, head->nq);
+        break;
+      } catch (const std::exception &e) {  // checked at submission; fail it rather than the worker
+        head->status = KATOANN_INVALID_ARGUMENT;
+        head->error = e.what();
+        head->done = true;
+        head->done_at = Clock::now();
+        queue.pop_front();
+        head = nullptr;
+        cv_done.notify_all();
+      }
     }
+    if (!head) return batch;
     if (reserved + need > opt.hbm_budget) return batch;  // wait for memory: no request overtakes the oldest
     batch.push_back(head);
     uint32_t nq = head->nq;
@@ -173,6 +181,7 @@ struct katoann_server {
       for (Request *r : batch) nq += r->nq;
       int status = KATOANN_OK;
       std::string error;
+      uint64_t used = 0;
       try {
         float *q = static_cast<float *>(qbuf.ensure(static_cast<size_t>(nq) * dim * 4));
         uint32_t *ids = static_cast<uint32_t *>(ibuf.ensure(static_cast<size_t>(nq) * h.p.k * 4));
@@ -182,10 +191,38 @@ struct katoann_server {
           std::memcpy(q + off, r->queries.data(), r->queries.size() * 4);
           off += r->queries.size();
         }
-        katoann_search_extras x{};
-        x.normalize = h.normalize;
-        x.nprobe_model = h.has_model ? &h.model : nullptr;
-        status = katoann_search_ex(ix, q, nq, &h.p, &x, ids, dists);
+        for (;;) {
+          // the search may not grow its buffers past the reservation: a pool
+          // overflow whose rerun needs more room returns, and the batch
+          // reserves that room (ahead of new batches) before running again
+          katoann_search_extras x{};
+          x.normalize = h.normalize;
+          x.nprobe_model = h.has_model ? &h.model : nullptr;
+          x.max_bytes = bytes;
+          uint64_t need = 0;
+          x.needed_bytes = &need;
+          status = katoann_search_ex(ix, q, nq, &h.p, &x, ids, dists);
+          if (status != KATOANN_RUNTIME_ERROR || need <= bytes) {
+            used = need;
+            break;
+          }
+          lk.lock();
+          reserved -= bytes;
+          bytes = 0;
+          if (need > opt.hbm_budget) {
+            lk.unlock();
+            katoann_error_slot() = "survivor pool overflow: the batch needs " + std::to_string(need) +
+                                   " bytes of HBM, above the budget of " + std::to_string(opt.hbm_budget);
+            break;
+          }
+          ++rerun_waiting;
+          cv_done.wait(lk, [&] { return reserved + need <= opt.hbm_budget; });
+          --rerun_waiting;
+          reserved += need;
+          peak_reserved = std::max(peak_reserved, reserved);
+          bytes = need;
+          lk.unlock();
+        }
         if (status != KATOANN_OK) {
           error = katoann_error_slot();
         } else {
@@ -205,6 +242,7 @@ struct katoann_server {
 
       lk.lock();
       reserved -= bytes;
+      peak_used = std::max(peak_used, used);
       --inflight;
       ++n_batches;
       batch_queries += nq;
@@ -384,6 +422,7 @@ int katoann_server_stats_get(katoann_server *s, katoann_server_stats *out) {
     st.achieved_qps = span > 0 ? static_cast<double>(s->n_queries) / span : 0.0;
     st.budget = s->opt.hbm_budget;
     st.peak_reserved = s->peak_reserved;
+    st.peak_used = s->peak_used;
     st.peak_inflight = s->peak_inflight;
     *out = st;
   });

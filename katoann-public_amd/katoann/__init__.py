@@ -58,7 +58,7 @@ class _Extras(C.Structure):
                 ("probes", C.c_void_p), ("probe_dists", C.c_void_p), ("survivors", C.c_void_p),
                 ("n_survivors", C.c_void_p), ("rotated", C.c_void_p), ("codes", C.c_void_p),
                 ("stream", C.c_void_p), ("nprobe_model", C.POINTER(NprobeModel)), ("nprobe_used", C.c_void_p),
-                ("coarse_only", C.c_int32)]
+                ("coarse_only", C.c_int32), ("max_bytes", C.c_uint64), ("needed_bytes", C.c_void_p)]
 
 
 class ServerOptions(C.Structure):
@@ -70,7 +70,7 @@ class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("queries", C.c_uint64), ("batches", C.c_uint64),
                 ("p50_ms", C.c_double), ("p95_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
                 ("mean_batch", C.c_double), ("achieved_qps", C.c_double), ("budget", C.c_uint64),
-                ("peak_reserved", C.c_uint64), ("peak_inflight", C.c_uint32)]
+                ("peak_reserved", C.c_uint64), ("peak_inflight", C.c_uint32), ("peak_used", C.c_uint64)]
 
 
 _lib = None
@@ -275,7 +275,8 @@ class Index:
     def search_ex(self, queries, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold",
                   nprobe_per_query=None, normalize=None, want=()):
         """search() with optional per-query probe counts and extra outputs.
-        want: any of "probes", "probe_dists", "survivors", "rotated", "codes"."""
+        want: any of "probes", "probe_dists", "survivors", "rotated", "codes"; the extras dict
+        also holds "device_bytes", the bytes of the search's device buffers."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim != 2 or q.shape[1] != self.dim:
             raise ValueError("queries must be (n, dim) float32")
@@ -309,7 +310,10 @@ class Index:
             dc = (self.dim + 63) // 64 * 64
             extra["codes"] = np.empty((nq, dc), np.uint16)
             x.codes = _ptr(extra["codes"])
+        used = C.c_uint64()
+        x.needed_bytes = C.cast(C.byref(used), C.c_void_p)
         _check(lib().katoann_search_ex(self._h, _ptr(q), nq, C.byref(p), C.byref(x), _ptr(ids), _ptr(dists)))
+        extra["device_bytes"] = used.value  # the search's device buffers
         return ids, dists, extra
 
     def search_device(self, q_ptr: int, nq: int, ids_ptr: int, dists_ptr: int, k=10, nprobe=64,

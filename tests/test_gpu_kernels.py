@@ -81,3 +81,23 @@ def test_reference_kernel_test_passes():
     r = subprocess.run([REF_KERNEL_TEST], capture_output=True, text=True, timeout=240)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0 and "all kernel checks passed" in r.stdout
+
+
+REF_INDEX_TEST = os.path.join(common.ROOT, "oracle", "_ref", "test_index_gfx950")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_INDEX_TEST),
+                    reason="built by `make -C oracle ref` where the reference sources are")
+def test_reference_index_test_passes():
+    """The reference's own tests/test_index.cpp, compiled unchanged against
+    include/katoann/index.h and linked to libkatoann.so: build, load, recall of
+    the full scan, prefix and high-byte ranking with 4,000 of 40,000
+    candidates surviving equal to the full scan, local = threshold selection,
+    the per-query coarse search and bf16 precision, on the GPU engine."""
+    import subprocess
+    print("reference test_index", flush=True)
+    r = subprocess.run([REF_INDEX_TEST], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "all index checks passed" in r.stdout
+    assert "4,000 of 40,000 candidates surviving equals the full scan" in r.stdout

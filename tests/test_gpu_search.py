@@ -357,3 +357,80 @@ def test_concurrent_searches_match_sequential(mixture_index):
     for t in ts:
         t.join()
     assert not errors, errors
+
+
+@pytest.mark.parametrize("k,nprobe,S,ranking", [(10, 64, 2000, "prefix"), (10, 256, 4000, "prefix"),
+                                                (1500, 64, 1500, "prefix"), (10, 128, 4096, "high"),
+                                                (1200, 32, 3000, "prefix")])
+def test_survivors_above_1024_vs_oracle(mixture_index, k, nprobe, S, ranking):
+    """max(survivors, k) up to 4,096, as the reference allows (src/index.cpp:914):
+    the block-level pool update and refinement give the oracle's survivors and
+    results."""
+    prefix, base, q = mixture_index
+    print("survivors", k, nprobe, S, ranking, flush=True)
+    ix = load(prefix)
+    check_against_oracle(ix, _oracle(prefix), q[:300], k, nprobe, S, ranking=ranking)
+
+
+@pytest.mark.parametrize("k,nprobe", [(1100, 64), (4096, 256)])
+def test_full_ranking_k_above_1024_vs_oracle(mixture_index, k, nprobe):
+    """Full ranking keeps the k nearest of all candidates, k up to 4,096."""
+    prefix, base, q = mixture_index
+    print("full ranking k", k, nprobe, flush=True)
+    ix = load(prefix)
+    ids, dists, _ = ix.search_ex(q[:200], k=k, nprobe=nprobe, ranking="full", normalize=False)
+    o = _oracle(prefix).search(q[:200], k=k, nprobe=nprobe, ranking="full")
+    common.compare(ids, dists, o["ids"], o["dists"], (o["flags"] & common.tie_mask("full", "threshold")) != 0)
+
+
+def test_survivors_above_4096_refused(mixture_index):
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    with pytest.raises(ValueError, match="above 4096"):
+        ix.search(q[:2], k=10, survivors=4097)
+
+
+def test_device_per_query_nprobe_out_of_range(mixture_index):
+    """Per-query nprobe in device memory: an entry of 0 or above nprobe is
+    refused (the host path's std::invalid_argument), after kernels that only
+    read probed ranks; a valid array then searches as the host path does."""
+    import torch
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    nq, k = 64, 10
+    qd = torch.from_numpy(np.ascontiguousarray(q[:nq])).cuda()
+    ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+    for bad in (0, 33):
+        npq = torch.full((nq,), 8, dtype=torch.int32, device="cuda")
+        npq[5] = bad
+        print("device nprobe entry", bad, flush=True)
+        with pytest.raises(ValueError, match="per-query nprobe"):
+            ix.search_device(qd.data_ptr(), nq, ids.data_ptr(), dists.data_ptr(), k=k, nprobe=32,
+                             npq_ptr=npq.data_ptr())
+        torch.cuda.synchronize()
+    npq = torch.from_numpy(np.arange(1, nq + 1, dtype=np.int32) % 32 + 1).cuda()
+    ix.search_device(qd.data_ptr(), nq, ids.data_ptr(), dists.data_ptr(), k=k, nprobe=32, npq_ptr=npq.data_ptr())
+    torch.cuda.synchronize()
+    hi, hd, _ = ix.search_ex(q[:nq], k=k, nprobe=32, normalize=False,
+                             nprobe_per_query=npq.cpu().numpy().astype(np.uint32))
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), hi)
+    assert np.array_equal(dists.cpu().numpy().view(np.uint32), hd.view(np.uint32))
+
+
+def test_footprint_covers_search_after_overflow(mixture_index):
+    """The footprint a serving loop reserves covers the buffers a search of
+    that shape takes, also after an overflow rerun raised the shape's room."""
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ix.set_pool_allowance(8)
+    srv_like = katoann.Server(ix, max_inflight=1)
+    try:
+        for rep in range(3):
+            fp = srv_like.footprint(300, k=10, nprobe=16, survivors=32)
+            used = ix.search_ex(q[:300], k=10, nprobe=16, survivors=32, normalize=False)[2]["device_bytes"]
+            print("footprint", rep, fp, used, flush=True)
+            if rep > 0:
+                assert used <= fp
+    finally:
+        srv_like.close()

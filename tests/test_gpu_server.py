@@ -158,3 +158,29 @@ def test_refusals(served):
         ids, dists, _ = srv.wait(t)
         assert ids.shape == (4, 10)
         srv.drain()
+
+
+def test_overflow_rerun_reserves_its_room(served):
+    """A survivor pool too small for a band (8 slots of room): the search may
+    not grow past its batch's reservation, so the batch reserves the room its
+    rerun needs and runs again; results equal the searches alone, and the
+    reservation always covered the buffers a batch ran with."""
+    ix, q = served
+    ix.set_pool_allowance(8)
+    try:
+        reqs = [dict(queries=q[i * 200:(i + 1) * 200], k=10, nprobe=16, survivors=32, model=None) for i in range(6)]
+        alone = [_alone(ix, r) for r in reqs]
+        ix.set_pool_allowance(8)  # forget the room the searches alone found
+        srv = katoann.Server(ix, max_inflight=2, max_batch=200)
+        try:
+            print("server overflow reruns", flush=True)
+            got = _serve(srv, reqs, threads=2)
+            st = srv.stats()
+        finally:
+            srv.close()
+        for (ids, dists, _), (ai, ad) in zip(got, alone):
+            assert np.array_equal(ids, ai) and np.array_equal(dists.view(np.uint32), ad.view(np.uint32))
+        assert st["peak_used"] > 0 and st["peak_reserved"] >= st["peak_used"]
+        assert st["peak_reserved"] <= st["budget"]
+    finally:
+        ix.set_pool_allowance(0)
