@@ -153,6 +153,23 @@ typedef struct katoann_search_extras {
 int katoann_search_ex(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
                       const katoann_search_extras *x, uint32_t *ids, float *dists);
 
+/* Allocation-free search. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). katoann_search_ex with device
+ * pointers (x->device_io = 1, x required), run in a static device array of
+ * KATOANN_NOALLOC_SCRATCH_BYTES on each GPU instead of buffers of its own,
+ * on x->stream (NULL: the null stream); it calls no hipMalloc, hipFree or
+ * other allocating routine. A search whose buffers exceed the array (e.g.
+ * B x nlist x 4 bytes of coarse distances at large batches) is refused with
+ * KATOANN_INVALID_ARGUMENT; a survivor-pool overflow is rerun within the
+ * array, or refused with KATOANN_RUNTIME_ERROR. No autotuning, graph replay
+ * or stage timing. Calls on one GPU share its array: the library serialises
+ * them, and the array keeps the last call's contents. Results equal
+ * katoann_search_ex's. */
+#define KATOANN_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int katoann_search_noalloc(const katoann_index *h, const float *queries, uint32_t nq, const katoann_search_params *p,
+                           const katoann_search_extras *x, uint32_t *ids, float *dists);
+
 /* Load from arrays instead of files: vectors (n x dim, grouped by list in list
  * order; host memory, or device memory when vectors_on_device), ids in the same
  * order, list_sizes (nlist), centroids (nlist x dim) and the rotation (dim x dim,

@@ -100,6 +100,7 @@ def lib():
         L.katoann_load_arrays.argtypes = [P, u32, u32, u32, i32, P, P, P, P, i32, P, u32, C.POINTER(_LoadOptions)]
         L.katoann_search.argtypes = [P, P, u32, C.POINTER(_SearchParams), P, P]
         L.katoann_search_ex.argtypes = [P, P, u32, C.POINTER(_SearchParams), C.POINTER(_Extras), P, P]
+        L.katoann_search_noalloc.argtypes = [P, P, u32, C.POINTER(_SearchParams), C.POINTER(_Extras), P, P]
         for name, rt in [("dim", u32), ("nlist", u32), ("size", u32), ("normalized", i32), ("engine", i32),
                          ("precision", i32), ("prefix_dims", u32), ("rotated", i32), ("batched_coarse", i32),
                          ("memory_bytes", u64)]:
@@ -338,6 +339,21 @@ class Index:
             x.nprobe_model = C.pointer(model)
         _check(lib().katoann_search_ex(self._h, C.c_void_p(q_ptr), nq, C.byref(p), C.byref(x),
                                        C.c_void_p(ids_ptr or 0), C.c_void_p(dists_ptr or 0)))
+
+    def search_noalloc(self, q_ptr: int, nq: int, ids_ptr: int, dists_ptr: int, k=10, nprobe=64,
+                       ranking="prefix", survivors=32, selection="threshold", npq_ptr: int | None = None,
+                       normalize=False, stream: int | None = None):
+        """search_device() through the allocation-free entry point: the search runs in
+        the library's static per-GPU scratch (KATOANN_NOALLOC_SCRATCH_BYTES) and
+        allocates no device memory."""
+        p = self._params(k, nprobe, ranking, survivors, selection)
+        x = _Extras()
+        x.device_io = 1
+        x.normalize = int(normalize)
+        x.nprobe_per_query = npq_ptr
+        x.stream = stream
+        _check(lib().katoann_search_noalloc(self._h, C.c_void_p(q_ptr), nq, C.byref(p), C.byref(x),
+                                            C.c_void_p(ids_ptr or 0), C.c_void_p(dists_ptr or 0)))
 
     def load_arrays(self, dim, nlist, n, normalized, centroids, list_sizes, ids, vectors, rotation,
                     prefix_dims, vectors_device_ptr: int | None = None):
