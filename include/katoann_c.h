@@ -185,6 +185,10 @@ int katoann_index_scale(const katoann_index *h, float *lo, double *inv_step, dou
 /* The unsigned 16-bit codes of the loaded index (n x Dc in list order, nlist x Dc)
  * and the ids in list order; any pointer may be NULL. */
 int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *centroid_codes, uint32_t *ids);
+/* The same for the rows of the given lists only, list after list in the
+ * order given (codes: rows x Dc, ids: rows); either pointer may be NULL. */
+int katoann_export_lists(const katoann_index *h, const uint32_t *lists, uint32_t nl, uint16_t *codes,
+                         uint32_t *ids);
 
 /* Per-stage GPU times of the last search on this index, in ms:
  * [0] normalise + rotate + quantise, [1] coarse, [2] probe selection,

@@ -1368,6 +1368,37 @@ int katoann_export_codes(const katoann_index *h, uint16_t *codes, uint16_t *cent
   KATOANN_CATCH
 }
 
+int katoann_export_lists(const katoann_index *h, const uint32_t *lists, uint32_t nl, uint16_t *codes,
+                         uint32_t *ids) {
+  KATOANN_TRY
+  if (h->nlist == 0) throw logic_err("search before load");
+  if (h->precision == KATOANN_BF16) throw std::invalid_argument("a bf16 index holds no 16-bit codes");
+  const uint32_t dc = h->dc, d = h->dim;
+  uint64_t at = 0;
+  std::vector<int8_t> a, b;
+  for (uint32_t i = 0; i < nl; ++i) {
+    const uint32_t c = lists[i];
+    if (c >= h->nlist) throw std::invalid_argument("list index out of range");
+    const uint64_t r0 = h->list_off[c], n = h->list_off[c + 1] - r0;
+    if (n == 0) continue;
+    if (codes) {
+      a.resize(static_cast<size_t>(n) * dc);
+      b.resize(a.size());
+      HIPCHECK(hipMemcpy(a.data(), h->hi.as<int8_t>() + r0 * dc, a.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(b.data(), h->lo_plane.as<int8_t>() + r0 * dc, b.size(), hipMemcpyDeviceToHost));
+      for (uint64_t r = 0; r < n; ++r) {
+        uint16_t *o = codes + (at + r) * dc;
+        const int8_t *ha = a.data() + r * dc, *la = b.data() + r * dc;
+        for (uint32_t k = 0; k < d; ++k) o[k] = static_cast<uint16_t>(((ha[k] + 128) << 8) | (la[k] + 128));
+        for (uint32_t k = d; k < dc; ++k) o[k] = 0;
+      }
+    }
+    if (ids) HIPCHECK(hipMemcpy(ids + at, h->ids.as<uint32_t>() + r0, n * 4, hipMemcpyDeviceToHost));
+    at += n;
+  }
+  KATOANN_CATCH
+}
+
 int katoann_set_timing(katoann_index *h, int32_t on) {
   h->timing.store(on != 0);
   return KATOANN_OK;

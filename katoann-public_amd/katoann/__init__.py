@@ -110,6 +110,7 @@ def lib():
         L.katoann_list_sizes.argtypes = [P, P]
         L.katoann_index_scale.argtypes = [P, P, P, P]
         L.katoann_export_codes.argtypes = [P, P, P, P]
+        L.katoann_export_lists.argtypes = [P, P, u32, P, P]
         L.katoann_set_timing.argtypes = [P, i32]
         L.katoann_stage_times.argtypes = [P, P, i32]
         L.katoann_last_work.argtypes = [P, P, i32]
@@ -386,6 +387,24 @@ class Index:
         ids = np.empty(n, np.uint32)
         _check(lib().katoann_export_codes(self._h, _ptr(codes), _ptr(cc), _ptr(ids)))
         return codes, cc, ids
+
+    def centroid_codes(self):
+        dc = (self.dim + 63) // 64 * 64
+        cc = np.empty((self.nlist, dc), np.uint16)
+        _check(lib().katoann_export_codes(self._h, None, _ptr(cc), None))
+        return cc
+
+    def export_lists(self, lists):
+        """Codes (rows x Dc u16) and ids of the rows of the given lists, list
+        after list in the order given."""
+        lists = np.ascontiguousarray(lists, np.uint32)
+        sizes = self.list_sizes()
+        n = int(sizes[lists].astype(np.int64).sum())
+        dc = (self.dim + 63) // 64 * 64
+        codes = np.empty((n, dc), np.uint16)
+        ids = np.empty(n, np.uint32)
+        _check(lib().katoann_export_lists(self._h, _ptr(lists), lists.size, _ptr(codes), _ptr(ids)))
+        return codes, ids
 
     def set_timing(self, on: bool = True):
         _check(lib().katoann_set_timing(self._h, int(on)))

@@ -202,12 +202,17 @@ float l2_flat(const float *a, const float *b, uint32_t d) {
   return tree + tail;
 }
 
+// Each element is rotate_one of its row and basis row; rows go in blocks of 8
+// per basis row (the basis row stays in cache), every element computed alike.
 void rotate_rows(const float *in, float *out, size_t n, uint32_t d, const float *rot) {
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) {
-    const float *x = in + static_cast<size_t>(i) * d;
-    for (uint32_t j = 0; j < d; ++j)
-      out[static_cast<size_t>(i) * d + j] = rotate_one(x, rot + static_cast<size_t>(j) * d, d, j);
+  const int64_t nb = static_cast<int64_t>((n + 7) / 8);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t b = 0; b < nb; ++b) {
+    const size_t i0 = static_cast<size_t>(b) * 8, i1 = std::min(n, i0 + 8);
+    for (uint32_t j = 0; j < d; ++j) {
+      const float *r = rot + static_cast<size_t>(j) * d;
+      for (size_t i = i0; i < i1; ++i) out[i * d + j] = rotate_one(in + i * d, r, d, j);
+    }
   }
 }
 

@@ -248,18 +248,25 @@ def read_ref_dump(path: str):
 
 def ref_search(prefix, queries_fbin, k, nprobe, survivors, ranking="prefix", selection="local",
                batch=4096, out=None, coarse="batched"):
-    """Runs the reference Index::search (AVX-512 engine) through katoann_ref_search."""
+    """Runs the reference Index::search (AVX-512 engine) through katoann_ref_search.
+    nprobe: an int (returns ids, dists) or a list of them, searched after one
+    load (returns a list of (ids, dists))."""
     import tempfile
+    many = not isinstance(nprobe, (int, np.integer))
+    nps = list(nprobe) if many else [int(nprobe)]
     tmp = out or tempfile.mktemp(suffix=".bin")
-    subprocess.run([REF_SEARCH, "search", prefix, queries_fbin, tmp, str(k), str(nprobe),
+    subprocess.run([REF_SEARCH, "search", prefix, queries_fbin, tmp, str(k), ",".join(str(x) for x in nps),
                     str(survivors), ranking, selection, str(batch), "0", coarse], check=True)
     raw = np.fromfile(tmp, dtype=np.uint8)
     if out is None:
         os.remove(tmp)
-    nqk = raw.size // 8
-    ids = raw[:4 * nqk].view(np.uint32)
-    dists = raw[4 * nqk:].view(np.float32)
-    return ids, dists
+    res = []
+    block = raw.size // len(nps)
+    for i in range(len(nps)):
+        r = raw[i * block:(i + 1) * block]
+        nqk = r.size // 8
+        res.append((r[:4 * nqk].view(np.uint32), r[4 * nqk:].view(np.float32)))
+    return res if many else res[0]
 
 
 def ref_flat_order(prefix, queries_fbin, nq, nlist):

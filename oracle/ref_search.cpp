@@ -7,10 +7,11 @@
 //   katoann_ref_search rotate PREFIX OUT.bin
 //       the rotated base vectors (f32, list order) from the load path's
 //       rotation of every list (index.cpp:613-629)
-//   katoann_ref_search search PREFIX QUERIES.fbin OUT.bin k nprobe survivors
+//   katoann_ref_search search PREFIX QUERIES.fbin OUT.bin k nprobe[,nprobe...] survivors
 //                             ranking(full|high|prefix) selection(threshold|local)
 //                             [batch=4096] [normalize=0] [mode=batched|flat|bf16]
-//       ids (u32) and dists (f32), nq x k, from Index::search in batches;
+//       ids (u32) and dists (f32), nq x k, from Index::search in batches (one
+//       block of both per nprobe, in the order given, from one load);
 //       flat loads with LoadOptions::batched_coarse = false, bf16 with
 //       LoadOptions::precision = kBf16
 //   katoann_ref_search flat   PREFIX QUERIES.fbin OUT.bin
@@ -120,7 +121,15 @@ int cmd_search(int argc, char **argv) {
   const std::string prefix = argv[2], qpath = argv[3], out = argv[4];
   katoann::SearchParams p;
   p.k = static_cast<uint32_t>(std::stoul(argv[5]));
-  p.nprobe = static_cast<uint32_t>(std::stoul(argv[6]));
+  // nprobe: one value, or several separated by commas (one load, the searches
+  // one after another, outputs concatenated in that order)
+  std::vector<uint32_t> nprobes;
+  for (size_t at = 0, nx; at <= std::string(argv[6]).size(); at = nx + 1) {
+    const std::string v(argv[6]);
+    nx = v.find(',', at);
+    if (nx == std::string::npos) nx = v.size();
+    nprobes.push_back(static_cast<uint32_t>(std::stoul(v.substr(at, nx - at))));
+  }
   p.survivors = static_cast<uint32_t>(std::stoul(argv[7]));
   const std::string r = argv[8], s = argv[9];
   p.ranking = r == "full" ? katoann::Ranking::kFull
@@ -143,16 +152,19 @@ int cmd_search(int argc, char **argv) {
     // through the reference's own CLI when that step itself is under test.
     return 3;
   }
-  std::vector<uint32_t> ids(static_cast<size_t>(nq) * p.k);
-  std::vector<float> dists(ids.size());
-  for (uint32_t off = 0; off < nq; off += batch) {
-    const uint32_t b = std::min(batch, nq - off);
-    idx.search(q.data() + static_cast<size_t>(off) * d, b, p, ids.data() + static_cast<size_t>(off) * p.k,
-               dists.data() + static_cast<size_t>(off) * p.k);
-  }
   std::ofstream f(out, std::ios::binary);
-  put(f, ids.data(), ids.size() * 4);
-  put(f, dists.data(), dists.size() * 4);
+  for (uint32_t np : nprobes) {
+    p.nprobe = np;
+    std::vector<uint32_t> ids(static_cast<size_t>(nq) * p.k);
+    std::vector<float> dists(ids.size());
+    for (uint32_t off = 0; off < nq; off += batch) {
+      const uint32_t b = std::min(batch, nq - off);
+      idx.search(q.data() + static_cast<size_t>(off) * d, b, p, ids.data() + static_cast<size_t>(off) * p.k,
+                 dists.data() + static_cast<size_t>(off) * p.k);
+    }
+    put(f, ids.data(), ids.size() * 4);
+    put(f, dists.data(), dists.size() * 4);
+  }
   return f ? 0 : 1;
 }
 

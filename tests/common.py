@@ -44,7 +44,7 @@ def tie_mask(ranking, selection):
 
 
 def _golden_dirs():
-    return sorted(n for n in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, n)))
+    return sorted(n for n in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, n, "reference_outputs.npz")))
 
 
 def _golden_keys(name):
@@ -61,6 +61,30 @@ def flat_golden_names():
     return [n for n in _golden_dirs() if "flat_order" in _golden_keys(n)]
 
 
+_WIDE = {}
+
+
+def wide_files(name):
+    """A regenerated fixture's index files (tests/golden/wide_fixture.py),
+    written once per process under the temporary directory: (prefix, queries)."""
+    if name not in _WIDE:
+        import atexit
+        import shutil
+        import sys
+        import tempfile
+        sys.path.insert(0, GOLDEN)
+        import wide_fixture
+        d = tempfile.mkdtemp(prefix="katoann_%s_" % name)
+        atexit.register(shutil.rmtree, d, True)
+        prefix = os.path.join(d, "idx")
+        q = wide_fixture.write(name, prefix)
+        ref = np.load(os.path.join(GOLDEN, name, "reference_outputs.npz"))
+        got = wide_fixture.files_sha(prefix)
+        assert got == str(ref["files_sha"]), "regenerated %s differs from the files the reference ran on" % name
+        _WIDE[name] = (prefix, q)
+    return _WIDE[name]
+
+
 def golden(name):
     """(index prefix, queries, reference outputs, cases); a case is
     (k, nprobe, survivors, ranking, selection, mode), mode "batched", "flat"
@@ -69,7 +93,11 @@ def golden(name):
     from katoann import formats
     d = os.path.join(GOLDEN, name)
     ref = np.load(os.path.join(d, "reference_outputs.npz"))
-    q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+    if os.path.exists(os.path.join(d, "queries.fbin")):
+        q = formats.read_fbin(os.path.join(d, "queries.fbin"))
+        prefix = os.path.join(d, "idx")
+    else:
+        prefix, q = wide_files(name)
     cases = []
     for key in ref.files:
         if key.startswith("ids_"):
@@ -77,7 +105,15 @@ def golden(name):
             k, nprobe, S, ranking, selection = parts[:5]
             coarse = parts[5] if len(parts) > 5 else "batched"
             cases.append((int(k), int(nprobe), int(S), ranking, selection, coarse))
-    return os.path.join(d, "idx"), q, ref, sorted(cases)
+    return prefix, q, ref, sorted(cases)
+
+
+def oracle_mode(ox, mode):
+    """The oracle index in a case's mode: batched coarse search, per-query
+    ("flat") coarse search, or bf16 precision."""
+    ox.set_bf16(mode == "bf16")
+    ox.set_flat(mode in ("flat", "bf16"))
+    return ox
 
 
 def case_tag(case):

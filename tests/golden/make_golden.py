@@ -3,7 +3,7 @@
 Run in the container that holds the reference sources (they never travel):
 
     make -C oracle ref            # builds oracle/_ref/ from /root/reference
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]
 
 For each fixture: a small seeded index (written by this project's builder in
 the reference's file format, committed as data), its queries, the scale and
@@ -28,6 +28,8 @@ from katoann import builder, datasets, formats  # noqa: E402
 import oracle as O  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import wide_fixture  # noqa: E402
 
 # name: (n, nq, d, nlist, centres, seed)
 FIXTURES = {
@@ -94,6 +96,70 @@ def grid(nlist):
     return out
 
 
+def wide_grid(d, nlist):
+    """Cases of the embedding-width fixtures: prefix ranking (P = 256 / 768),
+    high-byte ranking where the engine serves it (Dc <= 1024), full ranking,
+    the per-query coarse search and bf16 precision."""
+    out = []
+    for ranking in ("prefix", "high") if d <= 1024 else ("prefix",):
+        for selection in ("local", "threshold"):
+            for nprobe in (1, 4, 16):
+                for k, S in ((10, 32), (10, 128), (32, 32)):
+                    out.append((k, min(nprobe, nlist), S, ranking, selection))
+    for nprobe in (1, 4):
+        for k in (10, 32):
+            out.append((k, nprobe, k, "full", "threshold"))
+    out += [(10, 4, 32, "prefix", "threshold", "flat"), (32, 8, 32, "prefix", "local", "flat"),
+            (10, 4, 10, "full", "threshold", "bf16")]
+    return out
+
+
+def reference_state(prefix, tmp):
+    """Scale and checksums of the reference's Index::load of prefix."""
+    dump = os.path.join(tmp, "state.dump")
+    subprocess.run([O.REF_SEARCH, "dump", prefix, dump], check=True)
+    R = O.read_ref_dump(dump)
+    codes = ((R["hi"].astype(np.uint16) << 8) | R["lo_plane"]).astype(np.uint16)
+    ccodes = ((R["cen_hi"].astype(np.uint16) << 8) | R["cen_lo"]).astype(np.uint16)
+    rot = os.path.join(tmp, "state.rot")
+    subprocess.run([O.REF_SEARCH, "rotate", prefix, rot], check=True)
+    rotated = np.fromfile(rot, np.float32)
+    os.remove(dump)
+    os.remove(rot)
+    return dict(lo=np.float32(R["lo"]), inv_step=np.float64(R["inv_step"]), step_sq=np.float64(R["step_sq"]),
+                prefix_dims=np.uint32(R["prefix_dims"]), codes_sha=sha(codes), ccodes_sha=sha(ccodes),
+                norm_sha=sha(R["norm"]), cnorm_sha=sha(R["cen_norm"]), rotated_sha=sha(rotated))
+
+
+def make_wide(name):
+    """A regenerated fixture (tests/golden/wide_fixture.py): only the
+    reference's outputs and the generated files' checksum are committed."""
+    import tempfile
+    n, nq, d, nlist, seed = wide_fixture.SPECS[name]
+    out_dir = os.path.join(HERE, name)
+    if os.path.isdir(out_dir):
+        shutil.rmtree(out_dir)
+    os.makedirs(out_dir)
+    tmp = tempfile.mkdtemp(dir="/tmp")
+    prefix = os.path.join(tmp, "idx")
+    q = wide_fixture.write(name, prefix)
+    qf = os.path.join(tmp, "queries.fbin")
+    formats.write_fbin(qf, q)
+    res = reference_state(prefix, tmp)
+    res["files_sha"] = wide_fixture.files_sha(prefix)
+    cases = wide_grid(d, nlist)
+    for case in cases:
+        k, nprobe, S, ranking, selection = case[:5]
+        mode = case[5] if len(case) > 5 else "batched"
+        ids, dists = O.ref_search(prefix, qf, k, nprobe, S, ranking, selection, coarse=mode)
+        res["ids_" + tag_of(case)] = ids.reshape(nq, k)
+        res["dists_" + tag_of(case)] = dists.reshape(nq, k)
+        print(name, tag_of(case), flush=True)
+    np.savez_compressed(os.path.join(out_dir, "reference_outputs.npz"), **res)
+    shutil.rmtree(tmp)
+    print(name, "cases", len(cases))
+
+
 def tag_of(case):
     return "_".join(str(x) for x in case)
 
@@ -102,8 +168,13 @@ def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def main():
+def main(names=()):
+    for name in wide_fixture.SPECS:
+        if not names or name in names:
+            make_wide(name)
     for name, (n, nq, d, nlist, centres, seed) in FIXTURES.items():
+        if names and name not in names:
+            continue
         base, q = datasets.mixture(n, nq, d=d, centres=centres, sigma=0.3, seed=seed)
         out_dir = os.path.join(HERE, name)
         if os.path.isdir(out_dir):
@@ -135,6 +206,8 @@ def main():
         np.savez_compressed(os.path.join(out_dir, "reference_outputs.npz"), **res)
         print(name, "cases", len(cases), "files", sorted(os.listdir(out_dir)))
     name, n, nq, d, nlist, seed = FLAT_FIXTURE
+    if names and name not in names:
+        return
     out_dir = os.path.join(HERE, name)
     if os.path.isdir(out_dir):
         shutil.rmtree(out_dir)
@@ -148,4 +221,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))  # fixture names (default: all)

@@ -40,8 +40,7 @@ def test_load_matches_reference(name):
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
     ixs = {"batched": load(prefix), "flat": load(prefix, batched_coarse=False), "bf16": load(prefix, precision="bf16")}
-    oxs = {"batched": O.OracleIndex.from_files(prefix), "flat": O.OracleIndex.from_files(prefix).set_flat(),
-           "bf16": O.OracleIndex.from_files(prefix).set_bf16()}
+    ox = O.OracleIndex.from_files(prefix)
     rows = 0
     for case in cases:
         k, nprobe, S, ranking, selection, coarse = case
@@ -49,7 +48,7 @@ def test_search_matches_reference(name):
         print("golden case", name, tag, flush=True)
         ids, dists, _ = ixs[coarse].search_ex(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S,
                                               selection=selection, normalize=False)
-        flags = oxs[coarse].search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
+        flags = common.oracle_mode(ox, coarse).search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S)["flags"]
         tie = (flags & common.tie_mask(ranking, selection)) != 0
         n_ok, _ = common.compare(ids, dists, ref["ids_" + tag], ref["dists_" + tag], tie)
         rows += n_ok

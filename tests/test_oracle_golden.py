@@ -33,14 +33,12 @@ def test_load_state_matches_reference(name):
 @pytest.mark.parametrize("name", common.golden_names())
 def test_search_matches_reference(name):
     prefix, q, ref, cases = common.golden(name)
-    batched = O.OracleIndex.from_files(prefix)
-    oxs = {"batched": batched, "flat": O.OracleIndex.from_files(prefix).set_flat(),
-           "bf16": O.OracleIndex.from_files(prefix).set_bf16()}
+    ox = O.OracleIndex.from_files(prefix)
     exact_rows = 0
     for case in cases:
         k, nprobe, S, ranking, selection, mode = case
         tag = common.case_tag(case)
-        ix = oxs[mode]
+        ix = common.oracle_mode(ox, mode)
         r_ids, r_d = ref["ids_" + tag], ref["dists_" + tag]
         # libstdc++ tie emulation: identical to the reference on every row of kLocal
         o = ix.search(q, k=k, nprobe=nprobe, ranking=ranking, survivors=S, libstdc_ties=True)

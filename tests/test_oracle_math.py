@@ -126,3 +126,44 @@ def test_search_vs_bruteforce(tmp_path, k, nprobe, S):
     o = ix.search(q, k=k, nprobe=nprobe, survivors=S)
     assert np.array_equal(o["ids"], bi)
     assert np.array_equal(o["dists"].view(np.uint32), bd.view(np.uint32))
+
+
+def normalize_closed_form(x, acc=np.float64, divide=False):
+    """The front ends' row normalisation (python/katoann_module.cpp:97-110,
+    tools/katoann.cpp:102-111): s = sum over k, in k order, of double(x_k)^2;
+    rows with s == 0 stay as they are; otherwise x_k *= float(1 / sqrt(s)) in
+    float. A product of two floats is exact in double, so a fused or unfused
+    accumulation gives the same s; np.cumsum adds in k order. acc and divide
+    plant mistakes: a float accumulator, or x_k / float(sqrt(s))."""
+    x = np.asarray(x, np.float32)
+    s = np.cumsum(x.astype(acc) * x.astype(acc), axis=1, dtype=acc)[:, -1].astype(np.float64)
+    out = x.copy()
+    nz = s > 0
+    if divide:
+        out[nz] = x[nz] / np.sqrt(s[nz]).astype(np.float32)[:, None]
+    else:
+        inv = (1.0 / np.sqrt(s[nz])).astype(np.float32)
+        out[nz] = x[nz] * inv[:, None]
+    return out
+
+
+def test_normalize_rows_closed_form():
+    """oracle_normalize_rows equals the reference's closed form bit for bit on
+    rows of every scale (zero rows, denormal entries, large entries, d = 1 to
+    3,072); a float accumulator or a division by the norm each differ from it
+    on some rows, so the check separates them."""
+    rng = np.random.default_rng(97)
+    for d in (1, 7, 128, 1024, 3072):
+        x = rng.standard_normal((400, d)).astype(np.float32)
+        x[0] = 0.0
+        x[1] *= np.float32(1e-40)  # denormal entries
+        x[2] *= np.float32(1e18)
+        x[3, : d // 2] = 0.0
+        x[4:100] *= rng.uniform(1e-6, 1e6, (96, 1)).astype(np.float32)
+        got = O.normalize_rows(x)
+        want = normalize_closed_form(x)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), d
+        assert np.array_equal(got[0], x[0])
+        if d >= 128:
+            assert not np.array_equal(normalize_closed_form(x, acc=np.float32).view(np.uint32), got.view(np.uint32))
+            assert not np.array_equal(normalize_closed_form(x, divide=True).view(np.uint32), got.view(np.uint32))
