@@ -223,7 +223,12 @@ int katoann_set_pool_allowance(katoann_index *h, uint32_t slots);
 /* Ranking kernel: 0 = block items (a list's queries staged into LDS, rows
  * streamed), 1 = wave items (a wave keeps up to 16 T rows of a list in
  * registers and streams up to queries_per_item of its queries), 2 = block items
- * of 4 x 16 T rows whose 4 waves share the queries through an LDS ring;
+ * of 4 x 16 T rows whose 4 waves share the queries through an LDS ring,
+ * 3 = query-stationary wave items (a wave keeps up to 64 of a list's queries in
+ * registers, 32 with variant 1, and streams the list's rows 16 at a time with
+ * several tiles in flight; queries_per_item gives its rows per item),
+ * 4 = both forms in one launch: lists with at most 64 of a band's queries
+ * query-stationary (256 rows per item), the others as kernel 1 (variant 0);
  * variant 0 or 1 picks the larger or smaller T; variant 2 (kernel 1 only) a
  * smaller T with query codes loaded two fragments ahead instead of one;
  * variant 3 (kernel 1 only) a larger T at one wave per SIMD.

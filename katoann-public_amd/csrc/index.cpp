@@ -560,7 +560,8 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
   uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
-  bool prefix, full, bf16, wave, ring;
+  bool prefix, full, bf16, wave, ring, qs, hyb;
+  uint32_t qsq, qsrt;  // hybrid items: slots of at most qsq queries as query-stationary items of qsrt rows
   uint64_t max_rows, allowance;
   size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
       codes, npq, mx16, mh, qbf, nsurv;
@@ -702,12 +703,21 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.shift = tc.shift;
   // a seed band below S rows could leave a query without a threshold
   z.seed = tc.seed ? std::max(tc.seed, z.S) : 0u;
-  z.wave = !z.full && tc.kernel >= 1;
+  z.wave = !z.full && (tc.kernel == 1 || tc.kernel == 2 || tc.kernel == 4);
   z.ring = !z.full && tc.kernel == 2;
-  z.variant = z.full ? 0u : tc.variant;
-  z.qc = z.wave || z.full ? tc.qc : katoann_hip::rank_queries_per_item(z.P);
+  z.qs = !z.full && tc.kernel == 3;
+  z.hyb = !z.full && tc.kernel == 4;
+  z.variant = z.full || z.hyb ? 0u : tc.variant;
+  z.qsq = z.hyb ? katoann_hip::rank_qs_queries(z.P, 0) : 0u;
+  z.qsrt = z.hyb ? 256u : 0u;
+  // the query-stationary kernel: its queries per item are fixed by the
+  // variant, and the tile's queries_per_item field gives its rows per item
+  z.qc = z.qs                ? katoann_hip::rank_qs_queries(z.P, z.variant)
+         : z.wave || z.full ? tc.qc
+                            : katoann_hip::rank_queries_per_item(z.P);
   z.rt = z.bf16   ? katoann_hip::rank_bf16_rows()
          : z.full ? katoann_hip::rank_full_rows()
+         : z.qs   ? std::max(16u, tc.qc / 16 * 16)
          : z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1)
                   : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
@@ -738,7 +748,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   if (ix.pool_allowance) z.allowance = std::min<uint64_t>(z.max_rows, ix.pool_allowance);
   z.akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
             uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) | (uint64_t(z.full) << 10) |
-                (uint64_t(z.bf16) << 11)};
+                (uint64_t(z.bf16) << 11) | (uint64_t(z.qs) << 12) | (uint64_t(z.hyb) << 13)};
   {
     // later searches of a shape start from the room its last overflow rerun needed
     auto &mix = const_cast<katoann_index &>(ix);
@@ -823,6 +833,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
   const uint32_t grid = static_cast<uint32_t>(device_cus()) *
                         (z.bf16              ? katoann_hip::rank_bf16_blocks_per_cu()
                          : full              ? katoann_hip::rank_full_blocks_per_cu()
+                         : z.qs || z.hyb     ? katoann_hip::rank_qs_blocks_per_cu(P, z.variant)
                          : z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant)
                                              : 2u);
   Workspace w{};
@@ -879,10 +890,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       return;
     }
     katoann_hip::launch_plan(w.dist, w.probes, w.probe_d, npq, caller_npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
-                             qc, rt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals, w.pool_cnt,
+                             qc, rt, z.qsq, z.qsrt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals, w.pool_cnt,
                              w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, s);
-    katoann_hip::launch_items(w.item_off, nslots, nlist, ix.list_off_d.as<uint64_t>(), w.gcnt, w.qoff, qc, rt,
-                              item_cap, z.seed, w.items, s);
+    katoann_hip::launch_items(w.item_off, nslots, nlist, ix.list_off_d.as<uint64_t>(), w.gcnt, w.qoff, qc, rt, z.qsq,
+                              z.qsrt, item_cap, z.seed, w.items, s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
@@ -900,7 +911,13 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                       ix.ids.as<uint32_t>(), nlist, w.qhi, w.qlo, dc, w.qnorm, w.thr, w.qlist,
                                       w.item_off, w.items, stride, ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s);
       const bool launched =
-          full     ? true
+          full   ? true
+          : z.hyb ? katoann_hip::launch_rank_hyb(band, 0, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank,
+                                                 qstride, qn, w.lbound, w.qlist, w.item_off, w.items, stride,
+                                                 ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
+          : z.qs ? katoann_hip::launch_rank_qs(band, z.variant, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank,
+                                               qstride, qn, w.lbound, w.qlist, w.item_off, w.items, stride,
+                                               ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
           : z.wave ? katoann_hip::launch_rank_wave(band, z.variant, z.ring, plane, xnorm, ix.ids.as<uint32_t>(), nlist,
                                                    P, qrank, qstride, qn, w.lbound, w.qlist, w.item_off, w.items,
                                                    stride, ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
@@ -949,7 +966,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       SearchCtx &c = *ctx;
       // everything a launch reads: shapes, scalars, the workspace, the stream
       std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
-                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, prefix, full, z.bf16, grid, ix.gen,
+                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, z.qs, z.hyb, prefix, full, z.bf16, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, caller_npq, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
                                    reinterpret_cast<uint64_t>(base), ws_bytes, z.surv, z.nsurv, z.codes, z.qpre,
@@ -1212,7 +1229,9 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
   double best = 1e30;
   TileConfig pick{1u, 0u, 1024u, 3u, 0u};
   auto trial = [&](const TileConfig &c) {
-    if (katoann_hip::rank_wave_rows(key[0], c.variant) == 0) return;
+    if (c.kernel == 3 ? katoann_hip::rank_qs_queries(key[0], c.variant) == 0
+                      : katoann_hip::rank_wave_rows(key[0], c.variant) == 0)
+      return;
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
     double dt = 1e30;
     for (int rep = 0; rep < 2; ++rep) {  // the better of two timed runs
@@ -1241,6 +1260,11 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
   for (uint32_t variant : {0u, 1u, 2u, 3u})
     for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift, seed});
   for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift, seed});
+  // the query-stationary kernel: rows per item
+  for (uint32_t variant : {0u, 1u})
+    for (uint32_t rows : {256u, 1024u, 4096u}) trial(TileConfig{3u, variant, rows, shift, seed});
+  // both forms per list (query-stationary for lists with few queries)
+  for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{4u, 0u, qc, shift, seed});
   std::lock_guard<std::mutex> g(mix.tune_mu);
   mix.tuned[key] = pick;
 }
@@ -1460,8 +1484,8 @@ int katoann_set_graphs(katoann_index *h, int32_t on) {
 
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item) {
   KATOANN_TRY
-  if (kernel > 2 || variant > 3 || (kernel != 1 && variant > 1))
-    throw std::invalid_argument("rank kernel is 0, 1 or 2; variant 0, 1 or (kernel 1) 2 or 3");
+  if (kernel > 4 || variant > 3 || (kernel != 1 && variant > 1) || (kernel == 4 && variant != 0))
+    throw std::invalid_argument("rank kernel is 0 to 4; variant 0, 1 or (kernel 1) 2 or 3, 0 for kernel 4");
   if (queries_per_item == 0 || queries_per_item > 65536)
     throw std::invalid_argument("queries per ranking item must be in 1..65536");
   std::lock_guard<std::mutex> g(h->mu);

@@ -1048,8 +1048,8 @@ __device__ inline uint64_t block_exscan(uint64_t v, uint64_t *tmp, uint64_t *tot
 // per block of 1024 slots: their sums.
 __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t qtiles, uint32_t nslots,
                                                       uint32_t nlist, const uint64_t *__restrict__ list_off,
-                                                      uint32_t qc, uint32_t rt, uint32_t seed, uint32_t *gcnt,
-                                                      uint32_t *nit, uint32_t *bsum) {
+                                                      uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
+                                                      uint32_t seed, uint32_t *gcnt, uint32_t *nit, uint32_t *bsum) {
   __shared__ uint64_t tmp[17];
   const uint32_t sl = blockIdx.x * 1024 + threadIdx.x;
   uint32_t g = 0, it = 0;
@@ -1064,7 +1064,9 @@ __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t q
     uint64_t first, end;
     katoann_hip::slot_rows(sl / nlist, list_off[c + 1] - list_off[c], seed, first, end);
     const uint32_t n = static_cast<uint32_t>(end - first);
-    it = (g && n) ? ((g + qc - 1) / qc) * ((n + rt - 1) / rt) : 0u;
+    it = !(g && n)                    ? 0u
+         : katoann_hip::slot_qs(g, qsq) ? (n + qsrt - 1) / qsrt
+                                        : ((g + qc - 1) / qc) * ((n + rt - 1) / rt);
     gcnt[sl] = g;
     nit[sl] = it;
   }
@@ -1310,9 +1312,9 @@ void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s) {
 
 void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, bool check_npq,
                  uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
-                 uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist,
-                 uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
-                 uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
+                 uint32_t qsq, uint32_t qsrt, uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
+                 uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr,
+                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
   const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
   const bool from_probes = inv_from_probes(nlist, np);
   const uint32_t qtiles = from_probes ? 1u : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
@@ -1332,7 +1334,7 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
     hipLaunchKernelGGL(k_inv_probe<false>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
                        list_off, part, nullptr, nullptr);
     // (one partial count per slot: the counts become the slots' cursors, at 0)
-    hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, seed,
+    hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
                        gcnt, nit, bsum);
     hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
     hipLaunchKernelGGL(k_inv_probe<true>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
@@ -1342,7 +1344,7 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   }
   inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                          nullptr, nullptr);
-  hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, seed,
+  hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
                      gcnt, nit, bsum);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
   inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,

@@ -11,6 +11,13 @@ namespace katoann_hip {
 // updates it with atomics from every CU).
 constexpr uint32_t kCntStride = 32;
 
+// Hybrid ranking items: a (band, list) slot of at most qsq queries (qsq > 0)
+// becomes query-stationary items of qsrt rows that hold all its queries, with
+// kQsItem set in the item's query count; the other slots are split into items
+// of qc queries and rt rows as usual.
+constexpr uint32_t kQsItem = 0x80000000u;
+__host__ __device__ inline bool slot_qs(uint32_t g, uint32_t qsq) { return qsq && g <= qsq; }
+
 void launch_normalize(float *x, uint32_t n, uint32_t d, hipStream_t s);
 void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float *Y, hipStream_t s);
 void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32_t P, float lo,
@@ -66,15 +73,15 @@ size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shif
 // state[2].
 void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d, const uint32_t *npq, bool check_npq,
                  uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
-                 uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off, uint32_t *qlist,
-                 uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
-                 uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
+                 uint32_t qsq, uint32_t qsrt, uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
+                 uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr,
+                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
 void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
-                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
-                  uint4 *items, hipStream_t s);
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
+                  uint32_t cap, uint32_t seed, uint4 *items, hipStream_t s);
 // Rows [first, end) of list c's n rows that slot band `band` ranks.
 __host__ __device__ inline void slot_rows(uint32_t band, uint64_t n, uint32_t seed, uint64_t &first, uint64_t &end) {
   first = 0;
@@ -99,6 +106,23 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
                       const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
                       uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
                       uint32_t grid, hipStream_t s);
+// Query-stationary ranking kernel: a wave holds up to rank_qs_queries(P,
+// variant) queries of a list as MFMA A operands and streams the item's rows
+// (16 at a time, several tiles in flight) as B operands; 0 if unsupported.
+uint32_t rank_qs_queries(uint32_t P, uint32_t variant);
+// Hybrid launch (both forms per item, kQsItem): the wave form's tile of
+// `variant`, the query-stationary form's variant 0.
+bool launch_rank_hyb(uint32_t band, uint32_t variant, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                     uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                     const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                     uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                     uint32_t grid, hipStream_t s);
+uint32_t rank_qs_blocks_per_cu(uint32_t P, uint32_t variant);
+bool launch_rank_qs(uint32_t band, uint32_t variant, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                    uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                    const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                    uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                    uint32_t grid, hipStream_t s);
 // Ranking over the full codes (exact distances, pools of the k smallest
 // keys): rows per item, and the launch of one band.
 uint32_t rank_full_rows();

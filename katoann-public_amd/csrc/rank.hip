@@ -88,8 +88,8 @@ struct RankShape {
 // (band, list) slot in item_off.
 __global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, uint32_t nlist,
                         const uint64_t *__restrict__ list_off, const uint32_t *__restrict__ gcnt,
-                        const uint32_t *__restrict__ qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
-                        uint4 *items) {
+                        const uint32_t *__restrict__ qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
+                        uint32_t cap, uint32_t seed, uint4 *items) {
   const uint32_t it = blockIdx.x * 256 + threadIdx.x;
   const uint32_t total = item_off[nslots];
   if (it >= total || it >= cap) return;
@@ -103,6 +103,16 @@ __global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, 
   uint64_t first, end;
   katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
   const uint64_t n = end - first;
+  const uint32_t local0 = it - item_off[slot];
+  if (katoann_hip::slot_qs(gcnt[slot], qsq)) {  // all the slot's queries, rows qsrt at a time
+    uint4 d;
+    d.x = qoff[slot];
+    d.y = gcnt[slot] | katoann_hip::kQsItem;
+    d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(local0) * qsrt);
+    d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(local0 + 1) * qsrt));
+    items[it] = d;
+    return;
+  }
   // the query chunks of one row tile are consecutive items: dealt to
   // neighbouring waves (one CU, then the next block's), they read the tile's
   // rows from HBM about once per L2 instead of once per chunk
@@ -718,95 +728,329 @@ struct QFrag {
   uint32_t q;
 };
 
+// One item of the wave form (rows in registers, queries streamed).
+template <int KC, int T, int NB>
+__device__ __forceinline__ void wave_item(const uint4 d, const WaveArgs &a, KeepStage &ks, uint32_t &n_kept) {
+  constexpr uint32_t P = 64 * KC;
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
+  const uint32_t *ql = a.qlist + d.x;
+  // query of lane c16 in fragment f (clamped into the item)
+  auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
+  // Query fragments stream through a ring of NB register buffers, codes
+  // loaded NB - 1 fragments ahead and ids NB ahead; the loop is unrolled by
+  // NB so no buffer is ever copied (a copy would wait for the loads just
+  // issued). Ids are loaded before the loads that follow them (the rows,
+  // a step's codes), so waiting for an id waits for nothing issued after
+  // it; fragments past the item load its last query again (unconditional
+  // loads leave no register copies). Norms and bounds stay raw until the
+  // fragment is ranked.
+  uint32_t qfirst[NB];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) qfirst[u] = qid_of(u);
+  // the item's rows: B operands, norms and ids
+  v4i b[T * KC];
+  int32_t xn[T];
+  uint32_t xid[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const uint32_t row = min(d.z + 16 * t + c16, d.w - 1);
+    const int8_t *rp = a.plane + static_cast<size_t>(row) * P + 16 * g16;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) b[t * KC + kc] = load16(rp + 64 * kc);
+    xn[t] = a.xnorm[row];
+    xid[t] = a.ids[row];
+  }
+  QFrag<KC> Q[NB];
+  auto load_q = [&](QFrag<KC> &B, uint32_t q) {
+    B.q = q;
+    const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) B.A[kc] = load16(src + 64 * kc);
+    B.qn = a.qnorm[q];
+    B.lb = a.lbound[q];
+  };
+  uint32_t qnext = qfirst[NB - 1];
+#pragma unroll
+  for (int u = 0; u + 1 < NB; ++u) load_q(Q[u], qfirst[u]);
+  v4i seed[T];
+  uint32_t vmask, oddm;
+  row_seeds<T>(xn, rows, seed, vmask, oddm);
+  // fragment f from buffer C; fragment f + NB - 1's codes into buffer N
+  auto step = [&](uint32_t f, const QFrag<KC> &C, QFrag<KC> &N) {
+    const uint32_t qa = qnext;
+    qnext = qid_of(f + NB);
+    load_q(N, qa);
+    // lane c16's query: H = floor((qn - L) / 2), or 2^30 (above every
+    // accumulator) past the item
+    const int32_t H = 16 * f + c16 < nq ? (C.qn - C.lb) >> 1 : (1 << 30);
+    v4i acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[0], b[t * KC], seed[t]);
+#pragma unroll
+    for (int kc = 1; kc < KC; ++kc)
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[kc], b[t * KC + kc], acc[t]);
+    // H of the lane's 4 query slots 4 g16 + r
+    int32_t H4[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) H4[r] = __shfl(H, 4 * g16 + r);
+    if (!fragment_passes<T>(acc, H4)) return;
+    const uint32_t qid = C.q;
+    const int32_t qn = C.qn;
+    n_kept += keep_exact<T>(
+        ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
+        [&](int j) { return __shfl(qn, j); });
+  };
+  for (uint32_t f0 = 0; f0 < nf; f0 += NB) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (f0 + u >= nf) break;
+      step(f0 + u, Q[u], Q[(u + NB - 1) % NB]);
+    }
+  }
+}
+
 template <int KC, int T, int NB>
 __global__ __launch_bounds__(256, (T * KC > 32 ? 1 : 2)) void k_rank_wave(WaveArgs a) {
-  constexpr uint32_t P = 64 * KC;
   __shared__ uint64_t s_key[4][2 * KCAP];
   __shared__ uint32_t s_meta[4][2 * KCAP];
-  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  const int lane = lane_id();
   KeepStage ks{s_key[threadIdx.x >> 6], s_meta[threadIdx.x >> 6], 0u, 0, 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
   // items dealt round-robin over the grid's waves: a shared queue counter
   // serialises every taker on one address
   const uint32_t nw = gridDim.x * 4;
-  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw) {
-    const uint4 d = a.items[it];
-    const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
-    const uint32_t *ql = a.qlist + d.x;
-    // query of lane c16 in fragment f (clamped into the item)
-    auto qid_of = [&](uint32_t f) { return ql[min(16 * f + c16, nq - 1)]; };
-    // Query fragments stream through a ring of NB register buffers, codes
-    // loaded NB - 1 fragments ahead and ids NB ahead; the loop is unrolled by
-    // NB so no buffer is ever copied (a copy would wait for the loads just
-    // issued). Ids are loaded before the loads that follow them (the rows,
-    // a step's codes), so waiting for an id waits for nothing issued after
-    // it; fragments past the item load its last query again (unconditional
-    // loads leave no register copies). Norms and bounds stay raw until the
-    // fragment is ranked.
-    uint32_t qfirst[NB];
-#pragma unroll
-    for (int u = 0; u < NB; ++u) qfirst[u] = qid_of(u);
-    // the item's rows: B operands, norms and ids
-    v4i b[T * KC];
-    int32_t xn[T];
-    uint32_t xid[T];
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const uint32_t row = min(d.z + 16 * t + c16, d.w - 1);
-      const int8_t *rp = a.plane + static_cast<size_t>(row) * P + 16 * g16;
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) b[t * KC + kc] = load16(rp + 64 * kc);
-      xn[t] = a.xnorm[row];
-      xid[t] = a.ids[row];
+  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw)
+    wave_item<KC, T, NB>(a.items[it], a, ks, n_kept);
+  keep_drain(ks, a);
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
+// --------------------------------------------------------------------------
+// Ranking pass of one probe-rank band, query-stationary form. A wave takes
+// items (list, up to 16 QF of the band's queries of the list, a row range),
+// dealt round-robin, keeps the item's queries in registers as MFMA A operands
+// (QF fragments of 16) and streams the item's rows through 16 at a time as B
+// operands, NR row tiles in flight: every row byte comes from HBM once per
+// item, and one item holds all the band's queries of a list that has at most
+// 16 QF of them. The filter, exact test and pool appends are k_rank_wave's
+// (one 16-row tile per step).
+template <int KC>
+struct RowTile {
+  v4i b[KC];
+  int32_t xn;
+  uint32_t id;
+};
+
+// Stage of a query-stationary wave: the kept keys of its current item in LDS,
+// each with its query slot (0 .. 16 QF - 1, slot s = lane s's query) and its
+// index among the slot's staged keys, and per slot the count staged so far.
+// The queries stay the same for the whole item, so the stage is flushed (one
+// returning atomic per slot reserves its run in the query's pool, then the
+// keys are written) only when a fragment's keys would not fit, and at the end
+// of the item: the row stream carries no atomic whose return it waits for.
+constexpr uint32_t QS_CAP = 512;
+
+struct QsStage {
+  uint64_t *key;   // QS_CAP
+  uint32_t *meta;  // QS_CAP: slot | index << 6
+  uint32_t *cnt;   // 64: keys staged per slot
+  uint32_t n;      // keys staged (wave-uniform)
+};
+
+// Lane s (< 64) holds the query of slot s (slotq) and whether the slot is in
+// the item.
+__device__ __forceinline__ void qs_flush(QsStage &st, const WaveArgs &a, uint32_t slotq, bool slotok) {
+  const int lane = lane_id();
+  wave_fence();
+  const uint32_t c = st.cnt[lane];
+  uint32_t base = 0;
+  if (slotok && c) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(slotq) * katoann_hip::kCntStride], c);
+  bool ovf = false;
+  for (uint32_t e0 = 0; e0 < st.n; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    const bool v = e < st.n;
+    const uint32_t m = v ? st.meta[e] : 0u;
+    const uint64_t key = v ? st.key[e] : 0ull;
+    const int slot = static_cast<int>(m & 63u);
+    const uint32_t pos = __shfl(base, slot) + (m >> 6);
+    const uint32_t q = __shfl(slotq, slot);
+    if (v) {
+      if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = key;
+      else ovf = true;
     }
-    QFrag<KC> Q[NB];
-    auto load_q = [&](QFrag<KC> &B, uint32_t q) {
-      B.q = q;
-      const int8_t *src = a.qhi + static_cast<size_t>(q) * a.dc + 16 * g16;
+  }
+  if (ovf) atomicOr(&a.state[0], 1u);
+  st.cnt[lane] = 0;
+  st.n = 0;
+  wave_fence();
+}
+
+// Stages the kept keys of fragment f (query slots 16 f .. 16 f + 15) of one
+// 16-row tile: lane c16's row, slots 4 g16 + r. Returns the keys staged.
+__device__ __forceinline__ uint32_t qs_keep(QsStage &st, const WaveArgs &a, const v4i &acc, int f, bool rowok,
+                                            uint32_t odd, uint32_t id, const int32_t (&H4)[4], int32_t qnf,
+                                            uint32_t slotq, bool slotok) {
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  int32_t qnr[4];
 #pragma unroll
-      for (int kc = 0; kc < KC; ++kc) B.A[kc] = load16(src + 64 * kc);
-      B.qn = a.qnorm[q];
-      B.lb = a.lbound[q];
-    };
-    uint32_t qnext = qfirst[NB - 1];
+  for (int r = 0; r < 4; ++r) qnr[r] = __shfl(qnf, 4 * g16 + r);
+  uint32_t c = 0;
 #pragma unroll
-    for (int u = 0; u + 1 < NB; ++u) load_q(Q[u], qfirst[u]);
-    v4i seed[T];
-    uint32_t vmask, oddm;
-    row_seeds<T>(xn, rows, seed, vmask, oddm);
-    // fragment f from buffer C; fragment f + NB - 1's codes into buffer N
-    auto step = [&](uint32_t f, const QFrag<KC> &C, QFrag<KC> &N) {
-      const uint32_t qa = qnext;
-      qnext = qid_of(f + NB);
-      load_q(N, qa);
-      // lane c16's query: H = floor((qn - L) / 2), or 2^30 (above every
-      // accumulator) past the item
-      const int32_t H = 16 * f + c16 < nq ? (C.qn - C.lb) >> 1 : (1 << 30);
-      v4i acc[T];
+  for (int r = 0; r < 4; ++r) c |= static_cast<uint32_t>(rowok && acc[r] >= H4[r]) << (8 * r);
+  // per 16-lane group g, byte r: keys of slot 4 g + r up to this lane
+  const uint32_t inc = row_scan16(c), ex = inc - c;
+  const uint32_t G0 = __builtin_amdgcn_readlane(inc, 15), G1 = __builtin_amdgcn_readlane(inc, 31);
+  const uint32_t G2 = __builtin_amdgcn_readlane(inc, 47), G3 = __builtin_amdgcn_readlane(inc, 63);
+  const uint32_t S0 = byte_sum(G0), S1 = byte_sum(G1), S2 = byte_sum(G2), S3 = byte_sum(G3);
+  const uint32_t total = S0 + S1 + S2 + S3;
+  if (total == 0) return 0;
+  if (st.n + total > QS_CAP) qs_flush(st, a, slotq, slotok);
+  const uint32_t Gm = g16 == 0 ? G0 : g16 == 1 ? G1 : g16 == 2 ? G2 : G3;
+  uint32_t so = st.n + (g16 > 0 ? S0 : 0u) + (g16 > 1 ? S1 : 0u) + (g16 > 2 ? S2 : 0u);
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[0], b[t * KC], seed[t]);
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t slot = 16u * f + 4 * g16 + r;
+    const uint32_t prev = st.cnt[slot];  // keys of the slot staged before this fragment
+    if ((c >> (8 * r)) & 1u) {
+      const uint32_t idx = (ex >> (8 * r)) & 0xffu;
+      const int32_t l2 = qnr[r] - 2 * acc[r] - static_cast<int32_t>(odd);
+      st.key[so + idx] = make_key(fscale(l2, a.step_sq), id);
+      st.meta[so + idx] = slot | ((prev + idx) << 6);
+    }
+    so += (Gm >> (8 * r)) & 0xffu;
+  }
+  wave_fence();
+  // slot counts: lane c16 < 4 of group g16 adds slot 4 g16 + c16's keys
+  if (c16 < 4) {
+    const uint32_t slot = 16u * f + 4 * g16 + c16;
+    st.cnt[slot] += (Gm >> (8 * c16)) & 0xffu;
+  }
+  st.n += total;
+  wave_fence();
+  return total;
+}
+
+// One item of the query-stationary form (queries in registers, rows streamed).
+template <int KC, int QF, int NR>
+__device__ __forceinline__ void qs_item(const uint4 d, const WaveArgs &a, QsStage &st, uint32_t &n_kept) {
+  constexpr uint32_t P = 64 * KC;
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  const uint32_t nq = d.y, rows = d.w - d.z, ntile = (rows + 15) / 16, nf = (nq + 15) / 16;
+  const uint32_t *ql = a.qlist + d.x;
+  // tiles past the item load its last row again (unconditional loads)
+  auto load_rows = [&](RowTile<KC> &r, uint32_t t) {
+    const uint32_t row = d.z + min(16 * t + c16, rows - 1);
+    const int8_t *rp = a.plane + static_cast<size_t>(row) * P + 16 * g16;
 #pragma unroll
-      for (int kc = 1; kc < KC; ++kc)
+    for (int kc = 0; kc < KC; ++kc) r.b[kc] = load16(rp + 64 * kc);
+    r.xn = a.xnorm[row];
+    r.id = a.ids[row];
+  };
+  // the item's queries: lane c16 of fragment f holds query 16 f + c16 (the
+  // last one again past the item). Their ids are loaded first, then the
+  // first row tiles, then the codes that need the ids: the ids' round trip
+  // overlaps the rows', and the codes' wait does not wait for the rows.
+  uint32_t qid[QF];
 #pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = mfma(C.A[kc], b[t * KC + kc], acc[t]);
-      // H of the lane's 4 query slots 4 g16 + r
-      int32_t H4[4];
+  for (int f = 0; f < QF; ++f) qid[f] = ql[min(16u * f + c16, nq - 1)];
+  // lane s: the query of slot s (and whether slot s is in the item)
+  const uint32_t slotq = ql[min(static_cast<uint32_t>(lane), nq - 1)];
+  const bool slotok = static_cast<uint32_t>(lane) < min(nq, 16u * QF);
+  RowTile<KC> R[NR];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) H4[r] = __shfl(H, 4 * g16 + r);
-      if (!fragment_passes<T>(acc, H4)) return;
-      const uint32_t qid = C.q;
-      const int32_t qn = C.qn;
-      n_kept += keep_exact<T>(
-          ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
-          [&](int j) { return __shfl(qn, j); });
-    };
-    for (uint32_t f0 = 0; f0 < nf; f0 += NB) {
+  for (int u = 0; u + 1 < NR; ++u) load_rows(R[u], u);
+  v4i A[QF][KC];
+  int32_t qnv[QF], lbv[QF], H4[QF][4];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        if (f0 + u >= nf) break;
-        step(f0 + u, Q[u], Q[(u + NB - 1) % NB]);
+  for (int f = 0; f < QF; ++f) {
+    const int8_t *src = a.qhi + static_cast<size_t>(qid[f]) * a.dc + 16 * g16;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) A[f][kc] = load16(src + 64 * kc);
+    qnv[f] = a.qnorm[qid[f]];
+    lbv[f] = a.lbound[qid[f]];
+  }
+#pragma unroll
+  for (int f = 0; f < QF; ++f) {
+    // H = floor((qn - L) / 2), or 2^30 (above every accumulator) past the item
+    const int32_t H = 16u * f + c16 < nq ? (qnv[f] - lbv[f]) >> 1 : (1 << 30);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) H4[f][r] = __shfl(H, 4 * g16 + r);
+  }
+  for (uint32_t t0 = 0; t0 < ntile; t0 += NR) {
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      const uint32_t t = t0 + u;
+      if (t >= ntile) break;
+      load_rows(R[(u + NR - 1) % NR], t + NR - 1);
+      const RowTile<KC> &C = R[u];
+      const int32_t sd = -((C.xn + 1) >> 1);
+      const v4i seed = {sd, sd, sd, sd};
+      const bool rowok = 16 * t + c16 < rows;
+      const uint32_t odd = static_cast<uint32_t>(C.xn & 1);
+      v4i acc[QF][1];
+#pragma unroll
+      for (int f = 0; f < QF; ++f) {
+        if (static_cast<uint32_t>(f) >= nf) break;  // fragments past the item's queries
+        acc[f][0] = mfma(A[f][0], C.b[0], seed);
+#pragma unroll
+        for (int kc = 1; kc < KC; ++kc) acc[f][0] = mfma(A[f][kc], C.b[kc], acc[f][0]);
       }
+#pragma unroll
+      for (int f = 0; f < QF; ++f) {
+        if (static_cast<uint32_t>(f) >= nf) break;
+        if (!fragment_passes<1>(acc[f], H4[f])) continue;
+        n_kept += qs_keep(st, a, acc[f][0], f, rowok, odd, C.id, H4[f], qnv[f], slotq, slotok);
+      }
+    }
+  }
+  if (st.n) qs_flush(st, a, slotq, slotok);
+}
+
+template <int KC, int QF, int NR>
+__global__ __launch_bounds__(256, 2) void k_rank_qs(WaveArgs a) {
+  __shared__ uint64_t s_key[4][QS_CAP];
+  __shared__ uint32_t s_meta[4][QS_CAP];
+  __shared__ uint32_t s_cnt[4][64];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  QsStage st{s_key[w], s_meta[w], s_cnt[w], 0u};
+  st.cnt[lane] = 0;
+  wave_fence();
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  uint32_t n_kept = 0;
+  const uint32_t nw = gridDim.x * 4;
+  for (uint32_t it = ib + blockIdx.x * 4 + w; it < ie; it += nw) qs_item<KC, QF, NR>(a.items[it], a, st, n_kept);
+  if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
+}
+
+// Both forms in one launch: an item whose query count carries kQsItem (its
+// list has at most 16 QF of the band's queries) runs query-stationary, the
+// others with the list's rows in registers and its queries streamed, so a
+// list's rows come from HBM once either way.
+template <int KC, int T, int NB, int QF, int NR>
+__global__ __launch_bounds__(256, 2) void k_rank_hyb(WaveArgs a) {
+  __shared__ uint64_t s_key[4][2 * KCAP];
+  __shared__ uint32_t s_meta[4][2 * KCAP];
+  __shared__ uint64_t q_key[4][QS_CAP];
+  __shared__ uint32_t q_meta[4][QS_CAP];
+  __shared__ uint32_t q_cnt[4][64];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  KeepStage ks{s_key[w], s_meta[w], 0u, 0, 0u, 0u};
+  QsStage st{q_key[w], q_meta[w], q_cnt[w], 0u};
+  st.cnt[lane] = 0;
+  wave_fence();
+  const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
+  uint32_t n_kept = 0;
+  const uint32_t nw = gridDim.x * 4;
+  for (uint32_t it = ib + blockIdx.x * 4 + w; it < ie; it += nw) {
+    uint4 d = a.items[it];
+    if (d.y & katoann_hip::kQsItem) {
+      d.y &= ~katoann_hip::kQsItem;
+      qs_item<KC, QF, NR>(d, a, st, n_kept);
+    } else {
+      wave_item<KC, T, NB>(d, a, ks, n_kept);
     }
   }
   keep_drain(ks, a);
@@ -1728,10 +1972,10 @@ uint32_t rank_queries_per_item(uint32_t P) {
 }
 
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
-                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t cap, uint32_t seed,
-                  uint4 *items, hipStream_t s) {
+                  const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
+                  uint32_t cap, uint32_t seed, uint4 *items, hipStream_t s) {
   hipLaunchKernelGGL(k_items, dim3((cap + 255) / 256), dim3(256), 0, s, item_off, nslots, nlist, list_off, gcnt, qoff,
-                     qc, rt, cap, seed, items);
+                     qc, rt, qsq, qsrt, cap, seed, items);
 }
 
 template <int KC>
@@ -1842,6 +2086,83 @@ bool launch_rank_wave(uint32_t band, uint32_t variant, bool ring, const int8_t *
     W_CASE(9, 2, 2) W_CASE(9, 1, 2) W_CASE(10, 2, 2) W_CASE(10, 1, 2) W_CASE(11, 1, 2) W_CASE(12, 1, 2)
     W_CASE(13, 1, 2) W_CASE(14, 1, 2) W_CASE(15, 1, 2) W_CASE(16, 1, 2)
 #undef W_CASE
+    default:
+      return false;
+  }
+}
+
+// Query-stationary tiles per prefix chunk count KC = P / 64 and variant:
+// query fragments held (QF) and row tiles in flight (NR); the larger variant
+// first. Zero where the registers do not fit.
+struct QsTile {
+  uint8_t qf, nr;
+};
+static const QsTile kQsTiles[17][2] = {
+    {{0, 0}, {0, 0}}, {{4, 4}, {2, 4}}, {{4, 4}, {2, 4}}, {{4, 3}, {2, 3}}, {{4, 4}, {2, 3}}, {{2, 3}, {1, 3}},
+    {{2, 3}, {1, 3}}, {{2, 2}, {1, 3}}, {{2, 2}, {1, 2}}, {{1, 3}, {1, 2}}, {{1, 3}, {1, 2}}, {{1, 2}, {1, 2}},
+    {{1, 2}, {1, 2}}, {{1, 2}, {1, 2}}, {{1, 2}, {1, 2}}, {{1, 2}, {1, 2}}, {{1, 2}, {1, 2}}};
+
+uint32_t rank_qs_queries(uint32_t P, uint32_t variant) {
+  const uint32_t kc = P / 64;
+  if (kc == 0 || kc > 16 || P % 64 || variant > 1) return 0;
+  return 16u * kQsTiles[kc][variant].qf;
+}
+
+uint32_t rank_qs_blocks_per_cu(uint32_t, uint32_t) { return 2u; }  // the kernel's launch bounds
+
+template <int KC, int QF, int NR>
+static void qs_launch(const WaveArgs &a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_rank_qs<KC, QF, NR>), dim3(grid), dim3(256), 0, s, a);
+}
+
+bool launch_rank_qs(uint32_t band, uint32_t variant, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                    uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                    const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                    uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                    uint32_t grid, hipStream_t s) {
+  if (rank_qs_queries(P, variant) == 0) return false;
+  const QsTile q = kQsTiles[P / 64][variant];
+  const WaveArgs a{plane, xnorm, ids, qhi, dc, qnorm, lbound, qlist, item_off, items, nlist, stride, pool, pool_cnt,
+                   state, band, step_sq};
+  switch ((P / 64 * 8 + q.qf) * 8 + q.nr) {
+#define Q_CASE(kc, qf, nr)           \
+  case (kc * 8 + qf) * 8 + nr:       \
+    qs_launch<kc, qf, nr>(a, grid, s); \
+    return true;
+    Q_CASE(1, 4, 4) Q_CASE(1, 2, 4) Q_CASE(2, 4, 4) Q_CASE(2, 2, 4) Q_CASE(3, 4, 3) Q_CASE(3, 2, 3)
+    Q_CASE(4, 4, 4) Q_CASE(4, 2, 3) Q_CASE(5, 2, 3) Q_CASE(5, 1, 3) Q_CASE(6, 2, 3) Q_CASE(6, 1, 3)
+    Q_CASE(7, 2, 2) Q_CASE(7, 1, 3) Q_CASE(8, 2, 2) Q_CASE(8, 1, 2) Q_CASE(9, 1, 3) Q_CASE(9, 1, 2)
+    Q_CASE(10, 1, 3) Q_CASE(10, 1, 2) Q_CASE(11, 1, 2) Q_CASE(12, 1, 2) Q_CASE(13, 1, 2) Q_CASE(14, 1, 2)
+    Q_CASE(15, 1, 2) Q_CASE(16, 1, 2)
+#undef Q_CASE
+    default:
+      return false;
+  }
+}
+
+template <int KC, int T, int NB, int QF, int NR>
+static void hyb_launch(const WaveArgs &a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_rank_hyb<KC, T, NB, QF, NR>), dim3(grid), dim3(256), 0, s, a);
+}
+
+bool launch_rank_hyb(uint32_t band, uint32_t variant, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
+                     uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
+                     const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,
+                     uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
+                     uint32_t grid, hipStream_t s) {
+  if (variant != 0 || rank_wave_rows(P, 0) == 0 || rank_qs_queries(P, 0) == 0) return false;
+  const WaveArgs a{plane, xnorm, ids, qhi, dc, qnorm, lbound, qlist, item_off, items, nlist, stride, pool, pool_cnt,
+                   state, band, step_sq};
+  switch (P / 64) {
+#define H_CASE(kc, t, nb, qf, nr)            \
+  case kc:                                   \
+    hyb_launch<kc, t, nb, qf, nr>(a, grid, s); \
+    return true;
+    H_CASE(1, 8, 2, 4, 4) H_CASE(2, 8, 2, 4, 4) H_CASE(3, 6, 2, 4, 3) H_CASE(4, 6, 2, 4, 4) H_CASE(5, 4, 2, 2, 3)
+    H_CASE(6, 4, 2, 2, 3) H_CASE(7, 3, 2, 2, 2) H_CASE(8, 2, 2, 2, 2) H_CASE(9, 2, 2, 1, 3) H_CASE(10, 2, 2, 1, 3)
+    H_CASE(11, 1, 2, 1, 2) H_CASE(12, 1, 2, 1, 2) H_CASE(13, 1, 2, 1, 2) H_CASE(14, 1, 2, 1, 2)
+    H_CASE(15, 1, 2, 1, 2) H_CASE(16, 1, 2, 1, 2)
+#undef H_CASE
     default:
       return false;
   }

@@ -440,9 +440,11 @@ class Index:
     def set_rank_kernel(self, kernel: int, variant: int = 0, queries_per_item: int = 1024):
         """Ranking kernel: 0 = block items (queries staged into LDS), 1 = wave
         items (rows held in registers, queries streamed), 2 = block items with a
-        shared query ring; variant 0/1 = larger or smaller row tile per wave,
-        2 (kernel 1) = query codes loaded two fragments ahead, 3 (kernel 1) =
-        larger row tile at one wave per SIMD."""
+        shared query ring, 3 = query-stationary wave items (up to 64 of a list's
+        queries held in registers, 32 with variant 1, rows streamed;
+        queries_per_item is then the rows per item); variant 0/1 = larger or
+        smaller row tile per wave, 2 (kernel 1) = query codes loaded two
+        fragments ahead, 3 (kernel 1) = larger row tile at one wave per SIMD."""
         _check(lib().katoann_set_rank_kernel(self._h, kernel, variant, queries_per_item))
 
     def set_autotune(self, on: bool = True):
