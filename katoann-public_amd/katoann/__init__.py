@@ -199,11 +199,15 @@ class Index:
         _check(lib().katoann_index_create(C.byref(h)))
         self._h = h
 
-    def __del__(self):
+    def close(self):
+        """Frees the index's device memory now (the object is unusable after)."""
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
             _lib.katoann_index_destroy(h)
             self._h = None
+
+    def __del__(self):
+        self.close()
 
     # ---- reference API
     def load(self, prefix: str, precision: str = "sq16", use_amx: bool = True, prefix_dims: int = 0,

@@ -139,6 +139,27 @@ def zipf_mixture_torch(n: int, nq: int, d: int = 1024, comps: int = 16384, s: fl
     return draw(n), draw(nq)
 
 
+def zipf_queries_torch(nq: int, d: int = 1024, comps: int = 16384, s: float = 1.1, noise: float = 0.5,
+                       seed: int = 3, draw_seed: int = 101, device="cuda", chunk: int = 262144):
+    """More queries of zipf_mixture_torch(..., seed)'s law (same basis, means and
+    weights), drawn from their own seeded generator."""
+    import torch
+    g = _torch_gen(seed, device)
+    q = random_basis_torch(d, g, device).float()
+    sl = torch.sqrt(torch.arange(1, d + 1, dtype=torch.float64, device=device) ** -1.0).float()
+    means = (torch.randn((comps, d), generator=g, device=device) * sl) @ q.T
+    w = torch.arange(1, comps + 1, dtype=torch.float64, device=device) ** (-s)
+    w = w / w.sum()
+    g2 = _torch_gen(draw_seed, device)
+    out = torch.empty((nq, d), dtype=torch.float32, device=device)
+    for st in range(0, nq, chunk):
+        e = min(nq, st + chunk)
+        pick = torch.multinomial(w, e - st, replacement=True, generator=g2)
+        z = torch.randn((e - st, d), generator=g2, device=device, dtype=torch.float32)
+        out[st:e] = _normalize_torch(means[pick] + noise * ((z * sl) @ q.T))
+    return out
+
+
 def ground_truth_np(base: np.ndarray, queries: np.ndarray, k: int = 10, block: int = 256) -> np.ndarray:
     """Exact L2 top-k in float64, ties by id."""
     b = base.astype(np.float64)

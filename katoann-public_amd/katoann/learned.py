@@ -195,6 +195,9 @@ class NprobeMLP:
         return m
 
 
+MAX_LABEL = 4096
+
+
 def coarse(ix, queries, nprobe: int, chunk: int = 4096):
     """Probe lists and sorted coarse distances (device tensors, nq x nprobe) of
     queries (device float32, not normalised here), by the index's own coarse
@@ -217,14 +220,16 @@ def raw_features(pd, F: int):
     return x
 
 
-def coverage_labels(probes_full, list_of_id, gt_ids):
+def coverage_labels(probes, list_of_id, gt_ids, nlist=None):
     """Smallest nprobe covering the lists of each query's exact neighbours:
-    probes_full (nq x nlist, the whole coarse order), list_of_id (N), gt_ids (nq x k)."""
+    probes (nq x m, the first m lists of the coarse order, m = nlist for the
+    whole order), list_of_id (N), gt_ids (nq x k). A neighbour in a list past
+    the first m counts as at rank m (the label is capped at m)."""
     import torch
-    nq, nlist = probes_full.shape
-    pos = torch.empty_like(probes_full)
-    pos.scatter_(1, probes_full.long(), torch.arange(nlist, dtype=probes_full.dtype,
-                                                     device=probes_full.device).expand(nq, nlist).contiguous())
+    nq, m = probes.shape
+    nlist = nlist or m
+    pos = torch.full((nq, nlist), m - 1, dtype=probes.dtype, device=probes.device)
+    pos.scatter_(1, probes.long(), torch.arange(m, dtype=probes.dtype, device=probes.device).expand(nq, m).contiguous())
     nn_lists = list_of_id[gt_ids.long()].long()
     return torch.gather(pos, 1, nn_lists).max(1).values + 1
 
@@ -234,12 +239,13 @@ def train(ix, train_queries, list_of_id, gt_ids, nfeat=64, hidden=256, epochs=30
     """Build the training set with the index's coarse pass and fit a model."""
     import torch
     nlist = ix.nlist
-    probes, pd = coarse(ix, train_queries, nlist)
-    labels = coverage_labels(probes, list_of_id, gt_ids)
+    m = min(nlist, MAX_LABEL)  # the engine probes at most 4,096 lists per query
+    probes, pd = coarse(ix, train_queries, m)
+    labels = coverage_labels(probes, list_of_id, gt_ids, nlist)
     del probes
     model = NprobeMLP(nfeat, hidden, seed=seed, device=train_queries.device)
     model.set_scales(raw_features(pd, nfeat), labels)
-    X16 = model.features(pd, nlist)
+    X16 = model.features(pd, m)
     del pd
     torch.cuda.synchronize()
     hist = model.fit(X16, labels, epochs=epochs, seed=seed, log=log, tau=tau, lr=lr)
