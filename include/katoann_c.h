@@ -73,10 +73,46 @@ void katoann_default_build_options(katoann_build_options *o);
 void katoann_default_load_options(katoann_load_options *o);
 void katoann_default_search_params(katoann_search_params *p);
 
-/* Index::build (index.h:74-75, src/index.cpp:419-514): host k-means and PCA,
- * writes the index files under prefix. */
+/* Index::build (index.h:74-75, src/index.cpp:419-545): k-means partition and
+ * PCA rotation on the GPU (the vectors stream from host memory through pinned
+ * buffers), index files under prefix. The method is this library's own (see
+ * csrc/build.hip); the files, options and errors are the reference's. */
 int katoann_build(const float *data, uint32_t n, uint32_t dim, const katoann_build_options *opt,
                   const char *prefix);
+
+/* No counterpart in the reference (its build takes host vectors and writes
+ * files): the same build for vectors already in device memory, for callers
+ * whose data lives in HBM; katoann_load_arrays can load the result with no
+ * files. Every pointer is device memory except `rotated`: centroids (nlist x
+ * dim), order (n: the ids grouped by list, ascending inside each list), sizes
+ * (nlist), rotation (dim x dim, rows by decreasing variance; NULL skips the
+ * PCA); *rotated (host, may be NULL) = 1 when a rotation was computed. */
+int katoann_build_device(const float *data, uint32_t n, uint32_t dim, const katoann_build_options *opt,
+                         float *centroids, uint32_t *order, uint32_t *sizes, float *rotation, int32_t *rotated);
+
+/* The build's nearest-centroid assignment (the reference assigns every vector
+ * to its nearest centroid, src/index.cpp:457-463): n vectors (device, n x dim)
+ * against nlist centroids (device, nlist x dim), squared L2 in fp32, ties to
+ * the lowest index; out (device, n). */
+int katoann_assign_lists(const float *x, uint32_t n, uint32_t dim, const float *centroids, uint32_t nlist,
+                         uint32_t *out);
+
+/* The build's Lloyd iterations (the reference's k-means: src/kmeans.cpp:191-278)
+ * from the given centroids (device, nlist x dim, replaced by the result) over
+ * n training vectors (device): assignment as katoann_assign_lists, each centroid
+ * the mean of its vectors (fp64, ascending vector order), a centroid left
+ * without vectors unchanged. */
+int katoann_kmeans_device(const float *train, uint32_t n, uint32_t dim, float *centroids, uint32_t nlist,
+                          uint32_t iters);
+
+/* The build's rotation (the reference's PCA: src/index.cpp:276-380, 516-533)
+ * from m sample vectors (device, m x dim): rotation (device, dim x dim), row j
+ * the eigenvector of the j-th largest eigenvalue of the fp64 sample covariance,
+ * signed so its first largest-magnitude component is positive; eigenvalues
+ * (host, dim, decreasing; may be NULL); *rotated (host, may be NULL) = 0 and
+ * nothing written when m < 2 or dim < 2. */
+int katoann_pca_rotation_device(const float *sample, uint32_t m, uint32_t dim, float *rotation, double *eigenvalues,
+                                int32_t *rotated);
 
 int katoann_index_create(katoann_index **out);
 void katoann_index_destroy(katoann_index *h);

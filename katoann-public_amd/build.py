@@ -21,8 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "
           "-I" + os.path.join(HERE, "..", "include")]
 DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
-SOURCES = ["kernels.hip", "rank.hip", "mlp.hip", "index.cpp", "server.cpp", "noalloc.hip"]
-HOST_SOURCES = ["build.cpp"]  # host-only (index builder), g++ with OpenMP
+SOURCES = ["kernels.hip", "rank.hip", "mlp.hip", "index.cpp", "server.cpp", "noalloc.hip", "build.hip"]
 
 
 def _obj(src: str) -> str:
@@ -39,22 +38,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def _compile_host(src: str) -> str:
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    obj = _obj(src)
-    cmd = ["g++", "-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("compile failed: %s\n%s" % (" ".join(cmd), r.stderr))
-    return obj
-
-
 def build(jobs: int = 4) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_compile, s) for s in SOURCES] + [ex.submit(_compile_host, s) for s in HOST_SOURCES]
+        futs = [ex.submit(_compile, s) for s in SOURCES]
         objs = [f.result() for f in futs]
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))

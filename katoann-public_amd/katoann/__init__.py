@@ -128,6 +128,10 @@ def lib():
         L.katoann_normalize_rows.argtypes = [P, u64, u32]
         L.katoann_rotate_rows.argtypes = [P, P, u64, u32, P]
         L.katoann_build.argtypes = [P, u32, u32, C.POINTER(_BuildOptions), C.c_char_p]
+        L.katoann_build_device.argtypes = [P, u32, u32, C.POINTER(_BuildOptions), P, P, P, P, C.POINTER(i32)]
+        L.katoann_assign_lists.argtypes = [P, u32, u32, P, u32, P]
+        L.katoann_kmeans_device.argtypes = [P, u32, u32, P, u32, u32]
+        L.katoann_pca_rotation_device.argtypes = [P, u32, u32, P, P, C.POINTER(i32)]
         sz, f32 = C.c_size_t, C.c_float
         L.katoann_mlp_gemm.argtypes = [u32, u32, u32, P, sz, sz, P, sz, sz, P, P, sz, i32, P, P, sz, P]
         L.katoann_mlp_to_bf16.argtypes = [P, P, u64, P]
@@ -183,12 +187,45 @@ def amx_available() -> bool:
 
 
 def build(data, prefix: str, nlist: int = 2048, train_size: int = 256000, kmeans_iters: int = 12,
-          seed: int = 0, normalized: bool = False) -> None:
-    """Partition data (n x dim float32) and write the index files under prefix
-    (python/katoann_module.cpp:39-58). The base data is not normalised here."""
+          seed: int = 0, normalized: bool = False, rotation_sample: int = 100000) -> None:
+    """Partition data (n x dim float32) on the GPU and write the index files
+    under prefix (python/katoann_module.cpp:39-58). The base data is not
+    normalised here."""
     data = _f32(data)
-    o = _BuildOptions(nlist, train_size, kmeans_iters, seed, int(normalized), 100000, 8)
+    o = _BuildOptions(nlist, train_size, kmeans_iters, seed, int(normalized), rotation_sample, 8)
     _check(lib().katoann_build(_ptr(data), data.shape[0], data.shape[1], C.byref(o), prefix.encode()))
+
+
+def build_device(data_ptr: int, n: int, dim: int, centroids_ptr: int, order_ptr: int, sizes_ptr: int,
+                 rotation_ptr: int | None, nlist: int, train_size: int = 256000, kmeans_iters: int = 12,
+                 seed: int = 0, rotation_sample: int = 100000) -> bool:
+    """katoann_build_device on device buffers (addresses): data (n x dim f32)
+    -> centroids (nlist x dim f32), order (n u32), sizes (nlist u32), rotation
+    (dim x dim f32, or None to skip). Returns whether a rotation was computed."""
+    o = _BuildOptions(nlist, train_size, kmeans_iters, seed, 0, rotation_sample, 8)
+    r = C.c_int32(0)
+    _check(lib().katoann_build_device(data_ptr, n, dim, C.byref(o), centroids_ptr, order_ptr, sizes_ptr,
+                                      rotation_ptr, C.byref(r)))
+    return bool(r.value)
+
+
+def assign_lists(x_ptr: int, n: int, dim: int, centroids_ptr: int, nlist: int, out_ptr: int) -> None:
+    """katoann_assign_lists on device buffers: nearest of nlist centroids for
+    each of n vectors, ties to the lowest index, into out (n u32)."""
+    _check(lib().katoann_assign_lists(x_ptr, n, dim, centroids_ptr, nlist, out_ptr))
+
+
+def kmeans_device(train_ptr: int, n: int, dim: int, centroids_ptr: int, nlist: int, iters: int) -> None:
+    """katoann_kmeans_device: Lloyd iterations from the centroids in place."""
+    _check(lib().katoann_kmeans_device(train_ptr, n, dim, centroids_ptr, nlist, iters))
+
+
+def pca_rotation_device(sample_ptr: int, m: int, dim: int, rotation_ptr: int):
+    """katoann_pca_rotation_device: returns (rotated, eigenvalues decreasing)."""
+    eig = np.zeros(dim, np.float64)
+    r = C.c_int32(0)
+    _check(lib().katoann_pca_rotation_device(sample_ptr, m, dim, rotation_ptr, _ptr(eig), C.byref(r)))
+    return bool(r.value), eig
 
 
 class Index:

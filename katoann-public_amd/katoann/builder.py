@@ -1,17 +1,15 @@
 """Index builders that write the reference's file format.
 
-The reference builds with k-means++ seeding, Elkan k-means on a strided training
-sample, nearest-centre assignment and a Jacobi PCA rotation (src/index.cpp:419-533,
-src/kmeans.cpp). The search path only reads the files, so parity is judged on the
-same files whichever builder wrote them. These builders keep the same structure
-(strided sample, seeded initial centres, Lloyd iterations, full assignment with ids
-in ascending order inside each list, PCA basis sorted by decreasing variance) and
-are written for speed on their hardware:
+The search path only reads the files (or the arrays of ``load_arrays``), so
+parity is judged on the same files whichever builder wrote them.
 
-* ``build_numpy``: CPU, for test-sized data.
-* ``build_torch``: on the GPU with torch matrix products, for the million-scale
-  synthetic workloads (the k-means assignment is a GEMM + argmin; the PCA is
-  ``torch.linalg.eigh`` on the sample covariance).
+* ``build_torch`` / ``load_torch``: the library's GPU build (katoann_build_device,
+  csrc/build.hip) on vectors already in device memory: uniform training sample,
+  distinct training rows as initial centres, Lloyd iterations with the fp32 MFMA
+  assignment kernel and fp64 means, every vector assigned, ids ascending inside
+  each list, PCA rotation from rocSOLVER's eigensolver.
+* ``build_numpy``: a CPU builder of the same method for test-sized data where no
+  GPU is present (the CPU tests and the golden fixtures).
 """
 from __future__ import annotations
 
@@ -20,12 +18,11 @@ import numpy as np
 from . import formats
 
 
-def _strided_sample(n: int, ts: int) -> np.ndarray:
-    """Rows i * n / ts (src/index.cpp:433-443)."""
-    if ts >= n:
+def _sample(n: int, k: int, rng) -> np.ndarray:
+    """k distinct rows of n, uniform, ascending."""
+    if k >= n:
         return np.arange(n)
-    stride = (n << 32) // ts
-    return np.minimum((np.arange(ts, dtype=np.uint64) * np.uint64(stride)) >> np.uint64(32), n - 1).astype(np.int64)
+    return np.sort(rng.choice(n, k, replace=False))
 
 
 def _pca_rotation_np(sample: np.ndarray) -> np.ndarray:
@@ -33,8 +30,10 @@ def _pca_rotation_np(sample: np.ndarray) -> np.ndarray:
     x -= x.mean(axis=0, keepdims=True)
     cov = (x.T @ x) / x.shape[0]
     w, v = np.linalg.eigh(cov)
-    order = np.argsort(-w, kind="stable")
-    return np.ascontiguousarray(v[:, order].T.astype(np.float32))
+    rot = v[:, ::-1].T.copy()  # rows by decreasing eigenvalue
+    big = np.argmax(np.abs(rot), axis=1)
+    rot *= np.where(rot[np.arange(rot.shape[0]), big] < 0, -1.0, 1.0)[:, None]
+    return np.ascontiguousarray(rot.astype(np.float32))
 
 
 def _assign_np(data: np.ndarray, cen: np.ndarray, block: int = 16384) -> np.ndarray:
@@ -47,8 +46,7 @@ def _assign_np(data: np.ndarray, cen: np.ndarray, block: int = 16384) -> np.ndar
     return out
 
 
-def kmeans_np(train: np.ndarray, nlist: int, iters: int, seed: int) -> np.ndarray:
-    rng = np.random.default_rng(seed)
+def kmeans_np(train: np.ndarray, nlist: int, iters: int, rng) -> np.ndarray:
     cen = train[rng.choice(train.shape[0], nlist, replace=False)].astype(np.float64)
     for _ in range(iters):
         a = _assign_np(train, cen)
@@ -76,70 +74,47 @@ def build_numpy(data: np.ndarray, prefix: str, nlist: int = 2048, train_size: in
         raise ValueError("dimension must be 1 to 4096")
     if n == 0 or nlist == 0 or nlist > n:
         raise ValueError("need at least nlist vectors")
-    train = data[_strided_sample(n, min(train_size, n))]
-    cen = kmeans_np(train, nlist, kmeans_iters, seed)
+    rng = np.random.default_rng(seed)
+    train = data[_sample(n, max(min(train_size, n), nlist), rng)]
+    cen = kmeans_np(train, nlist, kmeans_iters, rng)
     lists = _lists(_assign_np(data, cen), nlist)
-    target = min(max(rotation_sample, 1024), n)
-    stride = max(1, n // target)
-    samp = data[np.concatenate(lists)[::stride][:target]]
+    samp = data[_sample(n, min(max(rotation_sample, 1024), n), rng)]
     rot = _pca_rotation_np(samp) if samp.shape[0] >= 2 and dim >= 2 else None
     formats.write_index(prefix, cen, lists, [data[ids] for ids in lists], n, normalized, rot)
 
 
 def build_torch(data, prefix: str | None, nlist: int, train_size: int = 256000,
                 kmeans_iters: int = 12, seed: int = 0, normalized: bool = False,
-                rotation_sample: int = 100000, block: int = 65536):
-    """GPU builder. ``data`` is an (n, dim) float32 torch tensor on the GPU.
+                rotation_sample: int = 100000):
+    """The GPU build (katoann_build_device) of an (n, dim) float32 CUDA tensor.
 
-    Returns dict(centroids, order, sizes, rotation) as torch tensors (order: row
-    ids grouped by list, ascending inside each list); writes files when
-    ``prefix`` is given."""
+    Returns dict(centroids, order, sizes, rotation) as CUDA tensors (order: row
+    ids grouped by list, ascending inside each list; rotation None when not
+    computed); writes the index files when ``prefix`` is given."""
     import torch
+    from . import build_device
 
+    data = data.contiguous()
     n, dim = data.shape
     dev = data.device
-    g = torch.Generator(device="cpu").manual_seed(int(seed))
-    tidx = torch.from_numpy(_strided_sample(n, min(train_size, n))).to(dev)
-    train = data.index_select(0, tidx)
-    init = torch.randperm(train.shape[0], generator=g)[:nlist].to(dev)
-    cen = train.index_select(0, init).clone()
-
-    def assign(x):
-        out = torch.empty(x.shape[0], dtype=torch.int64, device=dev)
-        for s in range(0, x.shape[0], block):
-            xb = x[s:s + block]
-            cn = (cen * cen).sum(1)
-            d = cn[None, :] - 2.0 * (xb @ cen.T)
-            out[s:s + block] = torch.argmin(d, dim=1)
-        return out
-
-    for _ in range(kmeans_iters):
-        a = assign(train)
-        sums = torch.zeros((nlist, dim), dtype=torch.float64, device=dev)
-        sums.index_add_(0, a, train.double())
-        cnt = torch.bincount(a, minlength=nlist).double()
-        nz = cnt > 0
-        cen[nz] = (sums[nz] / cnt[nz, None]).float()
-    a = assign(data)
-    order = torch.argsort(a, stable=True)
-    sizes = torch.bincount(a, minlength=nlist)
-    target = min(max(rotation_sample, 1024), n)
-    stride = max(1, n // target)
-    samp = data.index_select(0, order[::stride][:target]).double()
-    samp = samp - samp.mean(0, keepdim=True)
-    cov = (samp.T @ samp) / samp.shape[0]
-    w, v = torch.linalg.eigh(cov)
-    idx = torch.argsort(-w, stable=True)
-    rot = v[:, idx].T.contiguous().float()
-    out = dict(centroids=cen.contiguous(), order=order, sizes=sizes, rotation=rot)
+    cen = torch.empty((nlist, dim), dtype=torch.float32, device=dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    sizes = torch.empty(nlist, dtype=torch.int32, device=dev)
+    rot = torch.empty((dim, dim), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    rotated = build_device(data.data_ptr(), n, dim, cen.data_ptr(), order.data_ptr(), sizes.data_ptr(),
+                           rot.data_ptr(), nlist, train_size=train_size, kmeans_iters=kmeans_iters, seed=seed,
+                           rotation_sample=rotation_sample)
+    order = order.long()
+    out = dict(centroids=cen, order=order, sizes=sizes.long(), rotation=rot if rotated else None)
     if prefix is not None:
-        sz = sizes.cpu().numpy()
+        sz = out["sizes"].cpu().numpy()
         bounds = np.concatenate([[0], np.cumsum(sz)])
         o = order.cpu().numpy().astype(np.uint32)
         host = data.cpu().numpy()
         lists = [o[bounds[c]:bounds[c + 1]] for c in range(nlist)]
         formats.write_index(prefix, cen.cpu().numpy(), lists, [host[ids] for ids in lists], n,
-                            normalized, rot.cpu().numpy())
+                            normalized, rot.cpu().numpy() if rotated else None)
     return out
 
 
@@ -157,8 +132,9 @@ def load_torch(data, nlist: int, train_size: int = 256000, kmeans_iters: int = 1
     sizes = b["sizes"].cpu().numpy().astype(np.uint32)
     rows = data.index_select(0, order).contiguous()
     ix = Index()
+    rot = None if b["rotation"] is None else b["rotation"].cpu().numpy()
     ix.load_arrays(data.shape[1], nlist, data.shape[0], normalized, b["centroids"].cpu().numpy(), sizes,
-                   order.cpu().numpy().astype(np.uint32), None, b["rotation"].cpu().numpy(),
+                   order.cpu().numpy().astype(np.uint32), None, rot,
                    formats.default_prefix_dims(data.shape[1]), vectors_device_ptr=rows.data_ptr())
     torch.cuda.synchronize()
     del rows

@@ -93,6 +93,20 @@ def test_search_argument_checks():
     assert katoann.amx_available() is False
 
 
+def test_build_argument_checks(tmp_path):
+    """Index::build's checks (src/index.cpp:419-432), raised before any GPU work."""
+    p = str(tmp_path / "b")
+    with pytest.raises(ValueError, match="dimension must be 1 to 4096"):
+        katoann.build(np.zeros((10, 4097), np.float32), p, nlist=2)
+    with pytest.raises(ValueError, match="need at least nlist vectors"):
+        katoann.build(np.zeros((10, 8), np.float32), p, nlist=11)
+    with pytest.raises(ValueError, match="need at least nlist vectors"):
+        katoann.build(np.zeros((10, 8), np.float32), p, nlist=0)
+    with pytest.raises(ValueError, match="train_size is limited to 8388608"):
+        katoann.build(np.zeros(((1 << 23) + 1, 1), np.float32), p, nlist=2, train_size=(1 << 23) + 1)
+    assert not os.path.exists(p + ".meta")
+
+
 def test_server_needs_a_loaded_index():
     with pytest.raises(RuntimeError, match="not loaded"):
         katoann.Server(katoann.Index())
