@@ -29,10 +29,10 @@ struct BuildOptions {
   uint32_t nlist = 2048;
   uint32_t train_size = 256000;
   uint32_t kmeans_iters = 12;
-  uint64_t seed = 0;              // k-means++ seed; 0 picks a random seed
+  uint64_t seed = 0;              // seed of the samples and initial centres; 0 picks one
   bool normalized = false;        // record that the vectors are unit-length
   uint32_t rotation_sample = 100000;
-  uint32_t jacobi_sweeps = 8;
+  uint32_t jacobi_sweeps = 8;     // accepted; the eigensolver is rocSOLVER's divide and conquer
 };
 
 enum class Precision {

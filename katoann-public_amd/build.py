@@ -15,6 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkatoann.so")
+CLI = os.path.join(HERE, "bin", "katoann")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
@@ -48,7 +49,23 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, OUT)
+    _build_cli()
     return OUT
+
+
+def _build_cli() -> str:
+    """bin/katoann, the command-line front end, linked against libkatoann.so
+    found next to bin/ at run time."""
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "cli", "katoann.cpp"), "-o", tmp, "-L" + HERE, "-lkatoann",
+           "-Wl,-rpath,$ORIGIN/..", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":
