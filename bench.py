@@ -120,6 +120,14 @@ def run_workload(name, args, dev, headline):
         xp, xs = (int(x) for x in pt.split(":"))
         if xp <= W["nlist"] and (xp, xs) not in grid:
             grid.append((xp, xs))
+    if args.profile_point:  # one point, with the tile pinned (profiler runs)
+        grid = [tuple(int(x) for x in args.profile_point.split(":"))]
+        if args.tile:
+            kern, var, qc, shift, seed = (int(x) for x in args.tile.split(","))
+            ix.set_autotune(False)
+            ix.set_rank_kernel(kern, var, qc)
+            ix.set_band_factor(shift)
+            ix.set_seed_rows(seed)
     sweep, outs = sweep_points(ix, queries, gt, grid, sizes, pd, args)
     b99, b999 = best(sweep, 0.99), best(sweep, 0.999)
     head = b99 or max(sweep, key=lambda r: r["recall"])
@@ -129,6 +137,9 @@ def run_workload(name, args, dev, headline):
                sweep=[{kk: r[kk] for kk in ("nprobe", "survivors", "qps", "recall", "ms_per_step", "rank_gbs",
                                             "rank_tops", "tile")} for r in sweep])
     res["roofline"] = roofline_pass(ix, queries, head, sizes, pd, d, W["nlist"], name, args)
+    if args.profile_point:
+        ix.close()
+        return res
     learned_model = None
     if not args.no_learned and b99 and (headline or name == "s3072"):
         try:
@@ -265,7 +276,7 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
     (HBM gather) stages' fractions beside it."""
     import katoann
     nq, k = queries.shape[0], args.k
-    prof = profiled(name, head)
+    prof = None if args.profile_point else profiled(name, head)
     tile = head["tile"]
     if prof:
         kern, var, qc, shift, seed = prof["tile"]
@@ -581,6 +592,9 @@ def cpu_baseline(ix, base, queries, gt, b, sizes, pd, args, head):
 
 # ---------------------------------------------------------------- main
 
+EXTRA_KEYS = {"spectrum": "spectrum_1Mx1024", "s3072": "spectrum_990kx3072"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -604,6 +618,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
+    ap.add_argument("--profile-point", default="",
+                    help="NPROBE:S: measure only this point of the headline workload (profiler runs)")
+    ap.add_argument("--tile", default="", help="kernel,variant,queries_per_item,log2_band_factor,seed_rows to pin "
+                                               "with --profile-point")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, three device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (--gpus 1)", file=sys.stderr)
@@ -612,6 +632,17 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     res = run_workload(args.workload, args, dev, headline=True)
+    if args.profile_point:
+        if args.calib_copy_mib:
+            src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
+            for _ in range(3):
+                dst = src.clone()
+                torch.cuda.synchronize()
+                del dst
+        r = res["head"]
+        print(json.dumps({"profile_point": args.profile_point, "tile": r["tile"], "ms_per_step": r["ms_per_step"],
+                          "recall": r["recall"], "roofline": res["roofline"]}), flush=True)
+        return
     extras = {}
     for name in filter(None, args.extra.split(",")):
         if name == args.workload:
@@ -648,7 +679,7 @@ def main():
         "learned_nprobe": res.get("learned_nprobe"),
         "serving": res.get("serving"),
         "cpu_baseline": res.get("cpu_baseline"),
-        "extra": {("cfg2" if kk == "spectrum" else "cfg4" if kk == "s3072" else kk): v for kk, v in extras.items()},
+        "extra": {EXTRA_KEYS.get(kk, kk): v for kk, v in extras.items()},
     }
     print(json.dumps(out), flush=True)
 
