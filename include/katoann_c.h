@@ -234,6 +234,9 @@ int katoann_export_lists(const katoann_index *h, const uint32_t *lists, uint32_t
 #define KATOANN_NSTAGES 7
 int katoann_set_timing(katoann_index *h, int32_t on);
 int katoann_stage_times(const katoann_index *h, float *ms, int32_t n);
+/* Ranking-launch durations (ms) of each probe-rank band of the last timed
+ * search, band 0 first; 0 past its bands. */
+int katoann_band_times(const katoann_index *h, float *ms, int32_t n);
 /* Work of the last search: [0] (query, list) pairs, [1] ranking items,
  * [2] candidates kept by the ranking filter, [3] probe-rank bands. */
 int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n);
@@ -264,7 +267,12 @@ int katoann_set_pool_allowance(katoann_index *h, uint32_t slots);
  * registers, 32 with variant 1, and streams the list's rows 16 at a time with
  * several tiles in flight; queries_per_item gives its rows per item),
  * 4 = both forms in one launch: lists with at most 64 of a band's queries
- * query-stationary (256 rows per item), the others as kernel 1 (variant 0);
+ * query-stationary (256 rows per item), the others as kernel 1 (variant 0),
+ * 5 = block-tiled items (4 waves hold up to 256 of a list's queries, 128 above
+ * 256 prefix dims, in registers; the list's rows stream through an LDS ring of
+ * 64-row stages copied from HBM ahead of use; queries_per_item gives its rows
+ * per item; ranked widths up to 512 bytes, wider ones run kernel 1 with 1024
+ * queries per item);
  * variant 0 or 1 picks the larger or smaller T; variant 2 (kernel 1 only) a
  * smaller T with query codes loaded two fragments ahead instead of one;
  * variant 3 (kernel 1 only) a larger T at one wave per SIMD.

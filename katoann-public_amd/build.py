@@ -22,7 +22,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "
           "-I" + os.path.join(HERE, "..", "include")]
 DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
-SOURCES = ["kernels.hip", "rank.hip", "mlp.hip", "index.cpp", "server.cpp", "noalloc.hip", "build.hip"]
+SOURCES = ["kernels.hip", "rank.hip", "rank_gemm.hip", "mlp.hip", "index.cpp", "server.cpp", "noalloc.hip", "build.hip"]
 
 
 def _obj(src: str) -> str:

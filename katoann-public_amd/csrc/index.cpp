@@ -152,6 +152,7 @@ struct katoann_index {
   // last search's stage times and work, per calling thread
   std::mutex stat_mu;
   float stage_ms[KATOANN_NSTAGES] = {};
+  float band_ms[kMaxBands] = {};
   uint64_t work[4] = {};
 
   void reset() {
@@ -560,7 +561,7 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
   uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
-  bool prefix, full, bf16, wave, ring, qs, hyb;
+  bool prefix, full, bf16, wave, ring, qs, hyb, gemm;
   uint32_t qsq, qsrt;  // hybrid items: slots of at most qsq queries as query-stationary items of qsrt rows
   uint64_t max_rows, allowance;
   size_t q, qcodes, qpre, qnorm, q4, dist, probes, qlist, gcnt, pscratch, item_off, pool_cnt, thr, state, items, out, surv,
@@ -707,17 +708,24 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.ring = !z.full && tc.kernel == 2;
   z.qs = !z.full && tc.kernel == 3;
   z.hyb = !z.full && tc.kernel == 4;
-  z.variant = z.full || z.hyb ? 0u : tc.variant;
+  // the block-tiled kernel ranks widths up to 512 bytes; wider ones run kernel 1
+  z.gemm = !z.full && tc.kernel == 5 && katoann_hip::rank_gemm_queries(z.P) != 0;
+  if (!z.full && tc.kernel == 5 && !z.gemm) z.wave = true;
+  z.variant = z.full || z.hyb || z.gemm ? 0u : tc.variant;
   z.qsq = z.hyb ? katoann_hip::rank_qs_queries(z.P, 0) : 0u;
   z.qsrt = z.hyb ? 256u : 0u;
   // the query-stationary kernel: its queries per item are fixed by the
   // variant, and the tile's queries_per_item field gives its rows per item
+  // (and the block-tiled kernel's likewise, in stages of 64 rows)
   z.qc = z.qs                ? katoann_hip::rank_qs_queries(z.P, z.variant)
+         : z.gemm           ? katoann_hip::rank_gemm_queries(z.P)
+         : tc.kernel == 5 && !z.full ? 1024u
          : z.wave || z.full ? tc.qc
                             : katoann_hip::rank_queries_per_item(z.P);
   z.rt = z.bf16   ? katoann_hip::rank_bf16_rows()
          : z.full ? katoann_hip::rank_full_rows()
          : z.qs   ? std::max(16u, tc.qc / 16 * 16)
+         : z.gemm ? std::max(64u, tc.qc / 64 * 64)
          : z.wave ? katoann_hip::rank_wave_rows(z.P, z.variant) * (z.ring ? 4 : 1)
                   : ix.rank_rows;
   if (z.qc == 0 || z.rt == 0) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
@@ -748,7 +756,8 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   if (ix.pool_allowance) z.allowance = std::min<uint64_t>(z.max_rows, ix.pool_allowance);
   z.akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
             uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) | (uint64_t(z.full) << 10) |
-                (uint64_t(z.bf16) << 11) | (uint64_t(z.qs) << 12) | (uint64_t(z.hyb) << 13)};
+                (uint64_t(z.bf16) << 11) | (uint64_t(z.qs) << 12) | (uint64_t(z.hyb) << 13) |
+                (uint64_t(z.gemm) << 14)};
   {
     // later searches of a shape start from the room its last overflow rerun needed
     auto &mix = const_cast<katoann_index &>(ix);
@@ -834,6 +843,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                         (z.bf16              ? katoann_hip::rank_bf16_blocks_per_cu()
                          : full              ? katoann_hip::rank_full_blocks_per_cu()
                          : z.qs || z.hyb     ? katoann_hip::rank_qs_blocks_per_cu(P, z.variant)
+                         : z.gemm            ? katoann_hip::rank_gemm_blocks_per_cu(P)
                          : z.wave && !z.ring ? katoann_hip::rank_wave_blocks_per_cu(P, z.variant)
                                              : 2u);
   Workspace w{};
@@ -915,6 +925,9 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
           : z.hyb ? katoann_hip::launch_rank_hyb(band, 0, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank,
                                                  qstride, qn, w.lbound, w.qlist, w.item_off, w.items, stride,
                                                  ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
+          : z.gemm ? katoann_hip::launch_rank_gemm(band, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank,
+                                                   qstride, qn, w.lbound, w.qlist, w.item_off, w.items, stride,
+                                                   ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
           : z.qs ? katoann_hip::launch_rank_qs(band, z.variant, plane, xnorm, ix.ids.as<uint32_t>(), nlist, P, qrank,
                                                qstride, qn, w.lbound, w.qlist, w.item_off, w.items, stride,
                                                ix.step_sq, w.pool, w.pool_cnt, w.state, grid, s)
@@ -966,7 +979,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       SearchCtx &c = *ctx;
       // everything a launch reads: shapes, scalars, the workspace, the stream
       std::vector<uint64_t> key = {reinterpret_cast<uint64_t>(s), stride, B, d, dc, np, K, S, P, nlist, shift, z.seed, qc,
-                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, z.qs, z.hyb, prefix, full, z.bf16, grid, ix.gen,
+                                   rt, nbands, z.item_cap, z.variant, z.wave, z.ring, z.qs, z.hyb, z.gemm, prefix, full, z.bf16, grid, ix.gen,
                                    ix.rotated, x && x->normalize, coarse_only, caller_npq, reinterpret_cast<uint64_t>(npq),
                                    reinterpret_cast<uint64_t>(dids), reinterpret_cast<uint64_t>(ddst),
                                    reinterpret_cast<uint64_t>(base), ws_bytes, z.surv, z.nsurv, z.codes, z.qpre,
@@ -1088,9 +1101,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       float ms[6] = {};
       for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
       float rank = 0.0f;
-      for (uint32_t b = 0; b < nbands; ++b) {
+      for (uint32_t b = 0; b < kMaxBands; ++b) {
         float t = 0.0f;
-        (void)hipEventElapsedTime(&t, ctx->bev[2 * b], ctx->bev[2 * b + 1]);
+        if (b < nbands) (void)hipEventElapsedTime(&t, ctx->bev[2 * b], ctx->bev[2 * b + 1]);
+        mix.band_ms[b] = t;
         rank += t;
       }
       const float out[KATOANN_NSTAGES] = {ms[0], ms[1], ms[2], ms[3], rank, std::max(0.0f, ms[4] - rank), ms[5]};
@@ -1229,8 +1243,9 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
   double best = 1e30;
   TileConfig pick{1u, 0u, 1024u, 3u, 0u};
   auto trial = [&](const TileConfig &c) {
-    if (c.kernel == 3 ? katoann_hip::rank_qs_queries(key[0], c.variant) == 0
-                      : katoann_hip::rank_wave_rows(key[0], c.variant) == 0)
+    if (c.kernel == 3   ? katoann_hip::rank_qs_queries(key[0], c.variant) == 0
+        : c.kernel == 5 ? katoann_hip::rank_gemm_queries(key[0]) == 0
+                        : katoann_hip::rank_wave_rows(key[0], c.variant) == 0)
       return;
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
     double dt = 1e30;
@@ -1265,6 +1280,8 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
     for (uint32_t rows : {256u, 1024u, 4096u}) trial(TileConfig{3u, variant, rows, shift, seed});
   // both forms per list (query-stationary for lists with few queries)
   for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{4u, 0u, qc, shift, seed});
+  // the block-tiled kernel: rows per item
+  for (uint32_t rows : {512u, 2048u, 8192u}) trial(TileConfig{5u, 0u, rows, shift, seed});
   std::lock_guard<std::mutex> g(mix.tune_mu);
   mix.tuned[key] = pick;
 }
@@ -1435,6 +1452,13 @@ int katoann_stage_times(const katoann_index *h, float *ms, int32_t n) {
   return KATOANN_OK;
 }
 
+int katoann_band_times(const katoann_index *h, float *ms, int32_t n) {
+  auto *m = const_cast<katoann_index *>(h);
+  std::lock_guard<std::mutex> g(m->stat_mu);
+  for (int i = 0; i < n; ++i) ms[i] = i < static_cast<int>(kMaxBands) ? h->band_ms[i] : 0.0f;
+  return KATOANN_OK;
+}
+
 int katoann_last_work(const katoann_index *h, uint64_t *out, int32_t n) {
   auto *m = const_cast<katoann_index *>(h);
   std::lock_guard<std::mutex> g(m->stat_mu);
@@ -1484,8 +1508,8 @@ int katoann_set_graphs(katoann_index *h, int32_t on) {
 
 int katoann_set_rank_kernel(katoann_index *h, uint32_t kernel, uint32_t variant, uint32_t queries_per_item) {
   KATOANN_TRY
-  if (kernel > 4 || variant > 3 || (kernel != 1 && variant > 1) || (kernel == 4 && variant != 0))
-    throw std::invalid_argument("rank kernel is 0 to 4; variant 0, 1 or (kernel 1) 2 or 3, 0 for kernel 4");
+  if (kernel > 5 || variant > 3 || (kernel != 1 && variant > 1) || (kernel >= 4 && variant != 0))
+    throw std::invalid_argument("rank kernel is 0 to 5; variant 0, 1 or (kernel 1) 2 or 3, 0 for kernels 4 and 5");
   if (queries_per_item == 0 || queries_per_item > 65536)
     throw std::invalid_argument("queries per ranking item must be in 1..65536");
   std::lock_guard<std::mutex> g(h->mu);

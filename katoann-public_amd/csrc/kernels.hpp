@@ -118,6 +118,15 @@ bool launch_rank_hyb(uint32_t band, uint32_t variant, const int8_t *plane, const
                      uint32_t stride, double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state,
                      uint32_t grid, hipStream_t s);
 uint32_t rank_qs_blocks_per_cu(uint32_t P, uint32_t variant);
+// Block-tiled ranking kernel (rank_gemm.hip): queries per item (4 waves of
+// query tiles; 0 if unsupported, P above 512), blocks per CU, launch of one band.
+uint32_t rank_gemm_queries(uint32_t P);
+uint32_t rank_gemm_blocks_per_cu(uint32_t P);
+bool launch_rank_gemm(uint32_t band, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids, uint32_t nlist,
+                      uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm, const int32_t *lbound,
+                      const uint32_t *qlist, const uint32_t *item_off, const uint4 *items, uint32_t stride,
+                      double step_sq, uint64_t *pool, uint32_t *pool_cnt, uint32_t *state, uint32_t grid,
+                      hipStream_t s);
 bool launch_rank_qs(uint32_t band, uint32_t variant, const int8_t *plane, const int32_t *xnorm, const uint32_t *ids,
                     uint32_t nlist, uint32_t P, const int8_t *qhi, uint32_t dc, const int32_t *qnorm,
                     const int32_t *lbound, const uint32_t *qlist, const uint32_t *item_off, const uint4 *items,

@@ -114,6 +114,7 @@ def lib():
         L.katoann_set_timing.argtypes = [P, i32]
         L.katoann_stage_times.argtypes = [P, P, i32]
         L.katoann_last_work.argtypes = [P, P, i32]
+        L.katoann_band_times.argtypes = [P, P, i32]
         L.katoann_set_rank_rows.argtypes = [P, u32]
         L.katoann_set_band_factor.argtypes = [P, u32]
         L.katoann_set_seed_rows.argtypes = [P, u32]
@@ -454,6 +455,12 @@ class Index:
         out = np.zeros(NSTAGES, np.float32)
         _check(lib().katoann_stage_times(self._h, _ptr(out), NSTAGES))
         return dict(zip(STAGES, out.tolist()))
+
+    def band_times(self):
+        """Ranking-launch ms of each probe-rank band of the last timed search."""
+        out = np.zeros(16, np.float32)
+        _check(lib().katoann_band_times(self._h, _ptr(out), 16))
+        return out[:self.last_work()["bands"]].tolist()
 
     def last_work(self):
         out = np.zeros(4, np.uint64)

@@ -221,7 +221,7 @@ def wide_index(tmp_path_factory):
 RANK_KERNELS = [(0, 0, 512), (1, 0, 512), (1, 1, 512), (1, 2, 512), (1, 3, 512), (1, 0, 16), (1, 1, 1), (1, 2, 40),
                 (1, 2, 1), (1, 3, 24), (2, 0, 512), (2, 1, 512), (2, 0, 20), (2, 1, 1),
                 (3, 0, 1024), (3, 1, 1024), (3, 0, 16), (3, 1, 48), (3, 0, 4096), (4, 0, 512), (4, 0, 16),
-                (4, 0, 100)]
+                (4, 0, 100), (5, 0, 512), (5, 0, 64), (5, 0, 128), (5, 0, 8192)]
 
 
 _ORACLES = {}
@@ -300,7 +300,7 @@ def test_autotune_keeps_results(mixture_index):
         i, d, _ = ix.search_ex(q, k=10, nprobe=64, survivors=64, normalize=False)
         assert np.array_equal(i, ref[0]) and np.array_equal(d.view(np.uint32), ref[1].view(np.uint32))
     tuned = ix.tile_config(q.shape[0], k=10, nprobe=64, survivors=64)
-    assert tuned["kernel"] in (1, 2, 3, 4) and tuned["queries_per_item"] in (256, 512, 1024, 4096)
+    assert tuned["kernel"] in (1, 2, 3, 4, 5) and tuned["queries_per_item"] in (256, 512, 1024, 4096)
     print("default tile", default, "tuned", tuned, flush=True)
 
 
