@@ -533,16 +533,39 @@ def cpu_baseline(ix, base, queries, gt, b, sizes, pd, args, head):
     bounds = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))])
     keep = np.zeros(len(sizes), bool)
     keep[lists] = True
-    sel = torch.cat([order[bounds[c]:bounds[c + 1]] for c in lists]).long()
-    vec = base.index_select(0, sel).cpu().numpy()
     ids_all = order.cpu().numpy().astype(np.uint32)
+    # the load's scale comes from the smallest and largest rotated coordinate of
+    # the whole index: the rows that may hold them (the 64 smallest row minima
+    # and 64 largest row maxima of a float32 rotation) join the sub-index in
+    # their own lists, which the sample's queries do not probe when they are not
+    # probed lists already, so the sub-index quantises exactly as the full one
+    rot = b["rotation"]
+    lo_v, lo_i, hi_v, hi_i = [], [], [], []
+    for s0 in range(0, base.shape[0], 1 << 20):
+        y = base[s0:s0 + (1 << 20)] @ rot.T
+        mn, mx = y.min(1).values, y.max(1).values
+        v, i = torch.topk(mn, min(64, mn.numel()), largest=False)
+        lo_v.append(v), lo_i.append(i + s0)
+        v, i = torch.topk(mx, min(64, mx.numel()))
+        hi_v.append(v), hi_i.append(i + s0)
+        del y
+    lo_v, lo_i, hi_v, hi_i = (torch.cat(t) for t in (lo_v, lo_i, hi_v, hi_i))
+    ext = torch.cat([lo_i[torch.topk(lo_v, min(64, lo_v.numel()), largest=False).indices],
+                     hi_i[torch.topk(hi_v, min(64, hi_v.numel())).indices]]).cpu().numpy()
+    list_of = np.repeat(np.arange(len(sizes)), sizes.astype(np.int64))[np.argsort(ids_all, kind="stable")]
+    extra = np.unique(ext[~keep[list_of[ext]]]).astype(np.uint32)
     # the sub-index holds `rows` points: their ids renumbered 0..rows-1 in the
     # order of the original ids (ties between equal distances fall the same
     # way), the ground truth and the GPU's ids mapped alike (a neighbour
     # outside the sub-index maps to an id no search returns)
-    kept = np.sort(np.concatenate([ids_all[bounds[c]:bounds[c + 1]] for c in lists]))
+    kept = np.sort(np.concatenate([ids_all[bounds[c]:bounds[c + 1]] for c in lists] + [extra]))
     renum = np.full(base.shape[0], 0xFFFFFFFE, np.uint32)
     renum[kept] = np.arange(kept.size, dtype=np.uint32)
+    sel = torch.cat([order[bounds[c]:bounds[c + 1]] for c in lists]).long()
+    vec = base.index_select(0, sel).cpu().numpy()
+    xvec = base.index_select(0, torch.from_numpy(extra.astype(np.int64)).to(base.device)).cpu().numpy()
+    xlist = list_of[extra]
+    rows += int(extra.size)
     with tempfile.TemporaryDirectory(dir="/tmp") as dirn:
         prefix = os.path.join(dirn, "cpu")
         at = 0
@@ -553,13 +576,16 @@ def cpu_baseline(ix, base, queries, gt, b, sizes, pd, args, head):
                 lid.append(renum[ids_all[bounds[c]:bounds[c + 1]]])
                 lvec.append(vec[at:at + m])
                 at += m
-            else:
-                lid.append(np.zeros(0, np.uint32))
-                lvec.append(np.zeros((0, base.shape[1]), np.float32))
+            else:  # empty, or the extreme rows of an unprobed list (ascending ids)
+                xs = np.nonzero(xlist == c)[0]
+                lid.append(renum[extra[xs]])
+                lvec.append(xvec[xs].reshape(len(xs), base.shape[1]))
         t0 = time.time()
-        formats.write_index(prefix, b["centroids"].cpu().numpy(), lid, lvec, rows, True,
+        # written as not normalised: the CLI then ranks the queries as given,
+        # exactly as the GPU search above (normalize=False) did
+        formats.write_index(prefix, b["centroids"].cpu().numpy(), lid, lvec, rows, False,
                             b["rotation"].cpu().numpy(), pd)
-        del vec, lvec
+        del vec, lvec, xvec
         formats.write_fbin(os.path.join(dirn, "q.fbin"), host_q[:nq])
         formats.write_gt(os.path.join(dirn, "gt.bin"), renum[gt[:nq]])
         log("cpu baseline: %d queries, %d rows of %d lists written in %.1fs" % (nq, rows, len(lists), time.time() - t0))
@@ -582,7 +608,8 @@ def cpu_baseline(ix, base, queries, gt, b, sizes, pd, args, head):
                      np.all(c_d.view(np.uint32) == dists_g[:nq].view(np.uint32), 1)).sum())
     return {"value": res["qps"], "unit": "queries/s", "cores": threads, "kind": "reference",
             "sample": "the first %d of the %d queries in one batch (the reference CLI's --batch 4096), nprobe %d, "
-                      "survivors %d, on the %d lists (%d rows) they probe with every centroid kept; median of 3 "
+                      "survivors %d, on the %d lists they probe with every centroid kept (%d rows with the rows "
+                      "that fix the index's scale); median of 3 "
                       "after a warm-up; the full 41 GB of index files is not written within the benchmark's "
                       "run" % (nq, nq_all, np_, S, len(lists), rows),
             "recall_at_10": res.get("recall"), "rows_identical_to_gpu": same_rows, "rows_compared": nq,
