@@ -364,41 +364,53 @@ def learned_leg(ix, base, queries, gt, b, sizes, args, plain, more, ntrain, nlis
     gval = datasets.ground_truth_torch(base, val, 10)
     torch.cuda.synchronize()
     log("learned: %d training queries + fit %.1fs, loss %.3f -> %.3f" % (ntrain, time.time() - t0, hist[0], hist[-1]))
-    nmax = min(args.learned_nmax or W_nmax, nlist)
+    # the model's nprobe ceiling sets the probe selection's width (a top-nmax
+    # per query before the model runs): each candidate ceiling is calibrated on
+    # the validation queries, the fastest that reaches recall 0.99 is kept
+    ceilings = [args.learned_nmax] if args.learned_nmax else [c for c in (256, 512, W_nmax) if c <= W_nmax]
     used = torch.empty(nq, dtype=torch.int32, device=dev)
     ids_d = torch.empty((nq, args.k), dtype=torch.int32, device=dev)
     dists_d = torch.empty((nq, args.k), dtype=torch.float32, device=dev)
-
-    def run(qt, margin):
-        ix.search_device(qt.data_ptr(), nq, ids_d.data_ptr(), dists_d.data_ptr(), k=args.k, nprobe=nmax,
-                         survivors=S, model=model.struct(margin), npq_out_ptr=used.data_ptr())
-
-    margin = None
-    for mg in np.arange(-0.5, 4.01, 0.125):
-        run(val, float(mg))
-        if recall_at(ids_d.cpu().numpy().view(np.uint32), gval, 10) >= 0.99:
-            margin = float(mg)
-            break
-    if margin is None:
-        raise RuntimeError("no margin reaches recall 0.99 on the validation queries")
-    mstruct = model.struct(margin)
-    outs = [(torch.empty_like(ids_d), torch.empty_like(dists_d), torch.empty_like(used)) for _ in range(args.inflight)]
-
-    def run_on(j, n_):
-        for _ in range(n_):
-            ix.search_device(queries.data_ptr(), nq, outs[j][0].data_ptr(), outs[j][1].data_ptr(), k=args.k,
-                             nprobe=nmax, survivors=S, model=mstruct, npq_out_ptr=outs[j][2].data_ptr())
-
     from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(args.inflight) as pool:
-        list(pool.map(lambda j: run_on(j, args.warmup), range(args.inflight)))
-        torch.cuda.synchronize()
-        shares = [args.steps // args.inflight + (1 if j < args.steps % args.inflight else 0)
-                  for j in range(args.inflight)]
-        t0 = time.perf_counter()
-        list(pool.map(lambda j: run_on(j, shares[j]), range(args.inflight)))
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    best_fit = None
+    for nmax in sorted(set(min(c, nlist) for c in ceilings)):
+        def run(qt, margin, nmax=nmax):
+            ix.search_device(qt.data_ptr(), nq, ids_d.data_ptr(), dists_d.data_ptr(), k=args.k, nprobe=nmax,
+                             survivors=S, model=model.struct(margin), npq_out_ptr=used.data_ptr())
+
+        margin = None
+        for mg in np.arange(-0.5, 4.01, 0.125):
+            run(val, float(mg))
+            if recall_at(ids_d.cpu().numpy().view(np.uint32), gval, 10) >= 0.99:
+                margin = float(mg)
+                break
+        if margin is None:
+            log("learned: ceiling %d reaches no recall 0.99 on the validation queries" % nmax)
+            continue
+        mstruct = model.struct(margin)
+        outs = [(torch.empty_like(ids_d), torch.empty_like(dists_d), torch.empty_like(used))
+                for _ in range(args.inflight)]
+
+        def run_on(j, n_, nmax=nmax, mstruct=mstruct, outs=outs):
+            for _ in range(n_):
+                ix.search_device(queries.data_ptr(), nq, outs[j][0].data_ptr(), outs[j][1].data_ptr(), k=args.k,
+                                 nprobe=nmax, survivors=S, model=mstruct, npq_out_ptr=outs[j][2].data_ptr())
+
+        with ThreadPoolExecutor(args.inflight) as pool:
+            list(pool.map(lambda j: run_on(j, args.warmup), range(args.inflight)))
+            torch.cuda.synchronize()
+            shares = [args.steps // args.inflight + (1 if j < args.steps % args.inflight else 0)
+                      for j in range(args.inflight)]
+            t0 = time.perf_counter()
+            list(pool.map(lambda j: run_on(j, shares[j]), range(args.inflight)))
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        log("learned: ceiling %d margin %.3f: %.3f ms per batch" % (nmax, margin, ms))
+        if best_fit is None or ms < best_fit[0]:
+            best_fit = (ms, nmax, margin, mstruct, run)
+    if best_fit is None:
+        raise RuntimeError("no margin reaches recall 0.99 on the validation queries")
+    ms, nmax, margin, mstruct, run = best_fit
     ix.set_timing(True)
     stage_sum = np.zeros(len(katoann.STAGES))
     for _ in range(args.steps):
@@ -407,7 +419,8 @@ def learned_leg(ix, base, queries, gt, b, sizes, args, plain, more, ntrain, nlis
     ix.set_timing(False)
     rec = recall_at(ids_d.cpu().numpy().view(np.uint32), gt, 10)
     out = {"qps": nq / (ms / 1e3), "ms_per_step": ms, "recall": rec, "margin": margin, "survivors": S,
-           "nprobe_max": nmax, "mean_nprobe": float(used.float().mean().item()), "plain_nprobe": plain["nprobe"],
+           "nprobe_max": nmax, "ceilings_tried": sorted(set(min(c, nlist) for c in ceilings)),
+           "mean_nprobe": float(used.float().mean().item()), "plain_nprobe": plain["nprobe"],
            "plain_qps": plain["qps"], "speedup_vs_plain": (nq / (ms / 1e3)) / plain["qps"],
            "stages_ms": dict(zip(katoann.STAGES, (stage_sum / args.steps).tolist())),
            "training": "%d queries of the same law (own seed), coverage labels from the exact top-10, 20 epochs "

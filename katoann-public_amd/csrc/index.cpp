@@ -939,6 +939,9 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
                                                    w.pool, w.pool_cnt, w.state, grid, s);
       if (!launched) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
       if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band + 1], s));
+      // after the last band, k_refine (S <= 1,024) takes the S smallest keys
+      // of the whole pool itself: no update of its own
+      if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024) break;
       if (!katoann_hip::launch_update(w.pool, w.pool_cnt, w.pool_sorted, w.thr, w.lbound, B, S, stride, P, ix.step_sq,
                                       s))
         throw std::invalid_argument("unsupported survivor count");

@@ -1697,7 +1697,11 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   {
     const uint64_t base = static_cast<uint64_t>(q) * a.stride;
     const uint32_t m = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride);
-    const uint64_t tq = a.thr[q];
+    // keys above the threshold of the last update, or above the list's S-th
+    // once it is full, cannot be among the S smallest (the pool is not
+    // updated after the last band: it holds the S sorted survivors of the
+    // earlier bands, then the last band's keys)
+    uint64_t tq = a.thr[q];
     for (uint32_t b = 0; b < m; b += 64) {
       uint64_t k = b + lane < m ? a.pool[base + b + lane] : kNoKey;
       if (k > tq) k = kNoKey;
@@ -1713,6 +1717,7 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
       } else {
         top_merge<SC>(sv, k, a.S);
       }
+      tq = umin64(tq, top_get<SC>(sv, a.S - 1));
     }
   }
   const uint32_t ns = top_count<SC>(sv);
