@@ -688,6 +688,157 @@ __device__ __forceinline__ uint32_t keep_exact(KeepStage &ks, const WaveArgs &a,
   return total;
 }
 
+// --------------------------------------------------------------------------
+// Run-staged pool appends of one wave (the wave kernel). Kept keys stay in the
+// wave's LDS stage across fragments and items, each with the index of its
+// run: the keys one fragment keeps for one query, contiguous in the stage,
+// recorded as (query, first stage entry, length). The stage goes to the pools
+// only when it is full and at the wave's end: one returning atomic per run
+// reserves the run in its query's pool, then every key is written. So the
+// fragment loop issues no global store or atomic, and the compiler's waits
+// for the query fragments in flight count only those loads; the flush ends by
+// waiting for its own stores and atomics, so none is left for a later wait.
+constexpr uint32_t RS_KEYS = 1024;  // staged keys per wave
+constexpr uint32_t RS_RUNS = 256;   // staged runs per wave
+
+constexpr int vm_wait_imm(int n) { return (n & 0xf) | (((n >> 4) & 3) << 14) | (0x7 << 4) | (0xf << 8); }
+
+struct RunStage {
+  uint64_t *key;     // RS_KEYS
+  uint16_t *run;     // RS_KEYS: run of each key
+  uint32_t *rq;      // RS_RUNS: query of each run
+  uint32_t *rfirst;  // RS_RUNS: first stage entry (low 16 bits) | length << 16
+  uint32_t *rbase;   // RS_RUNS: pool position reserved at the flush
+  uint32_t n, nr;    // keys and runs staged (wave-uniform)
+};
+
+__device__ __forceinline__ void runs_flush(RunStage &rs, const WaveArgs &a) {
+  const int lane = lane_id();
+  wave_fence();
+  for (uint32_t i = lane; i < rs.nr; i += 64) {
+    const uint32_t f = rs.rfirst[i];
+    rs.rbase[i] = atomicAdd(&a.pool_cnt[static_cast<size_t>(rs.rq[i]) * katoann_hip::kCntStride], f >> 16);
+  }
+  wave_fence();
+  bool ovf = false;
+  for (uint32_t e = lane; e < rs.n; e += 64) {
+    const uint32_t r = rs.run[e];
+    const uint32_t pos = rs.rbase[r] + (e - (rs.rfirst[r] & 0xffffu));
+    if (pos < a.stride) a.pool[static_cast<size_t>(rs.rq[r]) * a.stride + pos] = rs.key[e];
+    else ovf = true;
+  }
+  if (ovf) atomicOr(&a.state[0], 1u);
+  rs.n = 0;
+  rs.nr = 0;
+  __builtin_amdgcn_s_waitcnt(vm_wait_imm(0));
+  wave_fence();
+}
+
+// keep_exact's filter and keys (same bit masks, counts and key formula), with
+// the keys staged as runs. A fragment with more keys than the stage holds (the
+// first band keeps every candidate) is written to the pools at once, after the
+// stage, and waited for.
+template <int T, class QS, class QN>
+__device__ __forceinline__ uint32_t keep_runs(RunStage &rs, const WaveArgs &a, const v4i (&acc)[T], uint32_t vmask,
+                                              uint32_t oddm, const uint32_t (&xid)[T], const int32_t (&H4)[4],
+                                              QS qslot, QN qnorm_slot) {
+  const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  int32_t qnr[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) qnr[r] = qnorm_slot(4 * g16 + r);
+  uint32_t bits[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int t = T - 1; t >= 0; --t)
+      m = __builtin_amdgcn_alignbit(m, static_cast<uint32_t>(H4[r] - 1 - acc[t][r]), 31);
+    bits[r] = m & vmask;
+  }
+  const uint32_t c = static_cast<uint32_t>(__builtin_popcount(bits[0])) |
+                     static_cast<uint32_t>(__builtin_popcount(bits[1])) << 8 |
+                     static_cast<uint32_t>(__builtin_popcount(bits[2])) << 16 |
+                     static_cast<uint32_t>(__builtin_popcount(bits[3])) << 24;
+  const uint32_t inc = row_scan16(c), ex = inc - c;
+  const uint32_t G0 = __builtin_amdgcn_readlane(inc, 15), G1 = __builtin_amdgcn_readlane(inc, 31);
+  const uint32_t G2 = __builtin_amdgcn_readlane(inc, 47), G3 = __builtin_amdgcn_readlane(inc, 63);
+  const uint32_t S0 = byte_sum(G0), S1 = byte_sum(G1), S2 = byte_sum(G2), S3 = byte_sum(G3);
+  const uint32_t total = S0 + S1 + S2 + S3;
+  if (total == 0) return 0;
+  const uint32_t Gm = g16 == 0 ? G0 : g16 == 1 ? G1 : g16 == 2 ? G2 : G3;
+  uint32_t so[4];  // first entry of slot 4 g16 + r within the fragment's keys
+  so[0] = (g16 > 0 ? S0 : 0u) + (g16 > 1 ? S1 : 0u) + (g16 > 2 ? S2 : 0u);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) so[r] = so[r - 1] + ((Gm >> (8 * (r - 1))) & 0xffu);
+  const uint32_t mine = c16 < 4 ? (Gm >> (8 * c16)) & 0xffu : 0u;
+  const uint32_t qmine = qslot(4 * g16 + (c16 & 3));  // every lane takes part in the shuffle
+  if (total > RS_KEYS) {
+    if (rs.n) runs_flush(rs, a);
+    uint32_t base = 0;
+    if (c16 < 4 && mine) base = atomicAdd(&a.pool_cnt[static_cast<size_t>(qmine) * katoann_hip::kCntStride], mine);
+    bool ovf = false;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t q = qslot(4 * g16 + r);
+      const uint32_t boff = __shfl(base, 16 * g16 + r);
+      uint32_t idx = (ex >> (8 * r)) & 0xffu;
+      int32_t av[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) av[t] = acc[t][r];
+      uint32_t m = bits[r];
+      while (m) {
+        const int t = __builtin_ctz(m);
+        m &= m - 1u;
+        const int32_t l2 = qnr[r] - 2 * pick(av, t) - static_cast<int32_t>((oddm >> t) & 1u);
+        const uint32_t pos = boff + idx;
+        if (pos < a.stride) a.pool[static_cast<size_t>(q) * a.stride + pos] = make_key(fscale(l2, a.step_sq), pick(xid, t));
+        else ovf = true;
+        ++idx;
+      }
+    }
+    if (ovf) atomicOr(&a.state[0], 1u);
+    __builtin_amdgcn_s_waitcnt(vm_wait_imm(0));
+    return total;
+  }
+  // slots with keys: bit 4 g + c of nz for slot 4 g + c; each gets a run
+  const uint64_t nzb = ballot(c16 < 4 && mine > 0);
+  uint32_t nz = 0;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) nz |= static_cast<uint32_t>((nzb >> (16 * g)) & 0xfu) << (4 * g);
+  const uint32_t nr = static_cast<uint32_t>(__builtin_popcount(nz));
+  if (rs.n + total > RS_KEYS || rs.nr + nr > RS_RUNS) runs_flush(rs, a);
+  if (c16 < 4 && mine) {
+    const uint32_t j = 4u * g16 + c16;
+    uint32_t sj = so[0];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) sj = c16 == r ? so[r] : sj;
+    const uint32_t ri = rs.nr + static_cast<uint32_t>(__builtin_popcount(nz & ((1u << j) - 1u)));
+    rs.rq[ri] = qmine;
+    rs.rfirst[ri] = (rs.n + sj) | (mine << 16);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t j = 4u * g16 + r;
+    const uint32_t ri = rs.nr + static_cast<uint32_t>(__builtin_popcount(nz & ((1u << j) - 1u)));
+    uint32_t e = rs.n + so[r] + ((ex >> (8 * r)) & 0xffu);
+    int32_t av[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) av[t] = acc[t][r];
+    uint32_t m = bits[r];
+    while (m) {
+      const int t = __builtin_ctz(m);
+      m &= m - 1u;
+      const int32_t l2 = qnr[r] - 2 * pick(av, t) - static_cast<int32_t>((oddm >> t) & 1u);
+      rs.key[e] = make_key(fscale(l2, a.step_sq), pick(xid, t));
+      rs.run[e] = static_cast<uint16_t>(ri);
+      ++e;
+    }
+  }
+  rs.n += total;
+  rs.nr += nr;
+  return total;
+}
+
 // Per item: the MFMA seeds -ceil(xn / 2) of the lane's rows, and the masks of
 // rows in the item and of odd norms.
 template <int T>
@@ -730,9 +881,13 @@ struct QFrag {
 
 // One item of the wave form (rows in registers, queries streamed).
 template <int KC, int T, int NB>
-__device__ __forceinline__ void wave_item(const uint4 d, const WaveArgs &a, KeepStage &ks, uint32_t &n_kept) {
+__device__ __forceinline__ void wave_item(const uint4 dv, const WaveArgs &a, RunStage &rs, uint32_t &n_kept) {
   constexpr uint32_t P = 64 * KC;
   const int lane = lane_id(), g16 = lane >> 4, c16 = lane & 15;
+  // the descriptor is the wave's (one item per wave): in scalar registers, the
+  // fragment loop's trip count and tests are uniform branches
+  const uint4 d{__builtin_amdgcn_readfirstlane(dv.x), __builtin_amdgcn_readfirstlane(dv.y),
+                __builtin_amdgcn_readfirstlane(dv.z), __builtin_amdgcn_readfirstlane(dv.w)};
   const uint32_t nq = d.y, nf = (nq + 15) / 16, rows = d.w - d.z;
   const uint32_t *ql = a.qlist + d.x;
   // query of lane c16 in fragment f (clamped into the item)
@@ -798,8 +953,8 @@ __device__ __forceinline__ void wave_item(const uint4 d, const WaveArgs &a, Keep
     if (!fragment_passes<T>(acc, H4)) return;
     const uint32_t qid = C.q;
     const int32_t qn = C.qn;
-    n_kept += keep_exact<T>(
-        ks, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
+    n_kept += keep_runs<T>(
+        rs, a, acc, vmask, oddm, xid, H4, [&](int j) { return static_cast<uint32_t>(__shfl(static_cast<int>(qid), j)); },
         [&](int j) { return __shfl(qn, j); });
   };
   for (uint32_t f0 = 0; f0 < nf; f0 += NB) {
@@ -813,18 +968,18 @@ __device__ __forceinline__ void wave_item(const uint4 d, const WaveArgs &a, Keep
 
 template <int KC, int T, int NB>
 __global__ __launch_bounds__(256, (T * KC > 32 ? 1 : 2)) void k_rank_wave(WaveArgs a) {
-  __shared__ uint64_t s_key[4][2 * KCAP];
-  __shared__ uint32_t s_meta[4][2 * KCAP];
-  const int lane = lane_id();
-  KeepStage ks{s_key[threadIdx.x >> 6], s_meta[threadIdx.x >> 6], 0u, 0, 0u, 0u};
+  __shared__ uint64_t s_key[4][RS_KEYS];
+  __shared__ uint16_t s_run[4][RS_KEYS];
+  __shared__ uint32_t s_rq[4][RS_RUNS], s_rfirst[4][RS_RUNS], s_rbase[4][RS_RUNS];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  RunStage rs{s_key[w], s_run[w], s_rq[w], s_rfirst[w], s_rbase[w], 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
   // items dealt round-robin over the grid's waves: a shared queue counter
   // serialises every taker on one address
   const uint32_t nw = gridDim.x * 4;
-  for (uint32_t it = ib + blockIdx.x * 4 + (threadIdx.x >> 6); it < ie; it += nw)
-    wave_item<KC, T, NB>(a.items[it], a, ks, n_kept);
-  keep_drain(ks, a);
+  for (uint32_t it = ib + blockIdx.x * 4 + w; it < ie; it += nw) wave_item<KC, T, NB>(a.items[it], a, rs, n_kept);
+  if (rs.n) runs_flush(rs, a);
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
@@ -1031,13 +1186,14 @@ __global__ __launch_bounds__(256, 2) void k_rank_qs(WaveArgs a) {
 // list's rows come from HBM once either way.
 template <int KC, int T, int NB, int QF, int NR>
 __global__ __launch_bounds__(256, 2) void k_rank_hyb(WaveArgs a) {
-  __shared__ uint64_t s_key[4][2 * KCAP];
-  __shared__ uint32_t s_meta[4][2 * KCAP];
+  __shared__ uint64_t s_key[4][RS_KEYS];
+  __shared__ uint16_t s_run[4][RS_KEYS];
+  __shared__ uint32_t s_rq[4][RS_RUNS], s_rfirst[4][RS_RUNS], s_rbase[4][RS_RUNS];
   __shared__ uint64_t q_key[4][QS_CAP];
   __shared__ uint32_t q_meta[4][QS_CAP];
   __shared__ uint32_t q_cnt[4][64];
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  KeepStage ks{s_key[w], s_meta[w], 0u, 0, 0u, 0u};
+  RunStage rs{s_key[w], s_run[w], s_rq[w], s_rfirst[w], s_rbase[w], 0u, 0u};
   QsStage st{q_key[w], q_meta[w], q_cnt[w], 0u};
   st.cnt[lane] = 0;
   wave_fence();
@@ -1050,10 +1206,10 @@ __global__ __launch_bounds__(256, 2) void k_rank_hyb(WaveArgs a) {
       d.y &= ~katoann_hip::kQsItem;
       qs_item<KC, QF, NR>(d, a, st, n_kept);
     } else {
-      wave_item<KC, T, NB>(d, a, ks, n_kept);
+      wave_item<KC, T, NB>(d, a, rs, n_kept);
     }
   }
-  keep_drain(ks, a);
+  if (rs.n) runs_flush(rs, a);
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
 
