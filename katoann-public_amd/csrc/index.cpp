@@ -943,7 +943,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       // of the whole pool itself: no update of its own
       if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024) break;
       if (!katoann_hip::launch_update(w.pool, w.pool_cnt, w.pool_sorted, w.thr, w.lbound, B, S, stride, P, ix.step_sq,
-                                      s))
+                                      z.bf16, s))
         throw std::invalid_argument("unsupported survivor count");
     }
     mark(5);

@@ -158,8 +158,12 @@ void launch_rank_bf16(uint32_t band, const uint16_t *rows, const float *xnorm, c
 void launch_emit_bf16(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, uint32_t B, uint32_t K,
                       uint32_t *out_ids, float *out_dists, uint64_t *surv, uint32_t *nsurv, hipStream_t s);
 uint32_t refine_cap(uint32_t S);
+// Pools back to their S smallest keys after a band, and the thresholds; with
+// keep_sorted false and S above 1,024 the kept keys stay in no particular
+// order (the refinement sorts them).
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
-                   uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s);
+                   uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, bool keep_sorted,
+                   hipStream_t s);
 bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_cnt, const uint64_t *thr,
                    uint32_t B, uint32_t S, uint32_t K, const int8_t *hi, const int8_t *lo, const int64_t *norm,
                    const uint32_t *id_row, uint32_t npts, uint32_t dc, const int8_t *qhi, const int8_t *qlo,

@@ -2023,6 +2023,90 @@ __global__ __launch_bounds__(256) void k_update_big(UpdateArgs a) {
   }
 }
 
+// k_update for S > 1,024 when no consumer needs the pool sorted: the S-th
+// smallest key of the pool by a most-significant-digit radix select (8-bit
+// digits of the u64 key; keys are unique), then the keys at or below it moved
+// to the front, in no particular order. The pool is read from global memory
+// for the histograms and moved through LDS UR_CHUNK keys at a time (a chunk
+// is read whole before any of it is written, and keys only move down).
+constexpr uint32_t UR_CHUNK = 8192;
+
+__global__ __launch_bounds__(256) void k_update_radix(UpdateArgs a) {
+  extern __shared__ uint64_t lds[];  // UR_CHUNK keys
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t stmp[4];
+  __shared__ uint32_t s_digit, s_rank, s_cnt, s_out;
+  __shared__ uint64_t s_pivot;
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t n = min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride);
+  const uint32_t sorted = a.pool_sorted[q] & 0x7fffffffu;
+  if (n == sorted) return;
+  if (n <= a.S) {  // every key stays; no threshold yet
+    if (tid == 0) a.pool_sorted[q] = n;
+    return;
+  }
+  uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
+  uint64_t prefix = 0, pmask = 0;
+  uint32_t rank = a.S;  // 1-based rank of the wanted key among those matching the prefix
+  bool unique = false;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += 256) {
+      const uint64_t k = pq[i];
+      if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    const uint32_t h = hist[tid];
+    const uint32_t ex = block_exscan256(h, stmp);
+    if (ex < rank && rank <= ex + h) {
+      s_digit = tid;
+      s_rank = rank - ex;
+      s_cnt = h;
+    }
+    __syncthreads();
+    prefix |= static_cast<uint64_t>(s_digit) << shift;
+    pmask |= 0xffull << shift;
+    rank = s_rank;
+    unique = s_cnt == 1;
+    __syncthreads();
+    if (unique) break;
+  }
+  uint64_t pivot = prefix;
+  if (unique) {  // the one key that matches the prefix
+    for (uint32_t i = tid; i < n; i += 256) {
+      const uint64_t k = pq[i];
+      if ((k & pmask) == prefix) s_pivot = k;
+    }
+    __syncthreads();
+    pivot = s_pivot;
+  }
+  // the keys at or below the pivot (exactly S) to the front
+  if (tid == 0) s_out = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < n; c0 += UR_CHUNK) {
+    const uint32_t len = min(UR_CHUNK, n - c0);
+    for (uint32_t i = tid; i < len; i += 256) lds[i] = pq[c0 + i];
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (uint32_t i = tid; i < len; i += 256) cnt += lds[i] <= pivot ? 1u : 0u;
+    uint32_t pos = s_out + block_exscan256(cnt, stmp);
+    for (uint32_t i = tid; i < len; i += 256)
+      if (lds[i] <= pivot) pq[pos++] = lds[i];
+    __syncthreads();
+    if (tid == 255) s_out = pos;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride] = a.S;
+    a.pool_sorted[q] = a.S;
+    if (pivot < a.thr[q]) {
+      a.thr[q] = pivot;
+      a.lbound[q] = static_cast<int32_t>(l2_bound(key_float(pivot), a.step_sq, a.lmax));
+    }
+  }
+}
+
 // k_refine for S > 1024: the pool after the last update holds the query's
 // survivors sorted ascending; the four waves take groups of 16 for the exact
 // distances; then the arrival rule in closed form (as in k_refine) over the
@@ -2043,15 +2127,18 @@ __global__ __launch_bounds__(256) void k_refine_big(RefineArgs a) {
   }
   __syncthreads();
   const uint32_t ns = block_count_sorted(buf, BIG_SB, stmp);
+  // the pool holds the survivors in no particular order (k_update_radix);
+  // they arrive in descending key order, so sort them ascending first
+  const uint64_t *sv = block_sort_keys(buf, BIG_SB);
   if (a.surv)
-    for (uint32_t i = tid; i < a.S; i += 256) a.surv[static_cast<size_t>(q) * a.S + i] = buf[i];
+    for (uint32_t i = tid; i < a.S; i += 256) a.surv[static_cast<size_t>(q) * a.S + i] = sv[i];
   if (a.nsurv && tid == 0) a.nsurv[q] = ns;
   const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
   const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
   const int64_t qn = a.qnorm[q];
   for (uint32_t t0 = 16 * w; t0 < ns; t0 += 64) {
     const uint32_t s = t0 + (lane & 15);
-    const uint64_t key = buf[min(s, BIG_SB - 1)];
+    const uint64_t key = sv[min(s, BIG_SB - 1)];
     const uint64_t e = exact_key16(a, key, s < ns, qh, ql, qn);
     if (lane < 16 && t0 + lane < ns) ex[t0 + lane] = e;
   }
@@ -2361,9 +2448,19 @@ void launch_rank_full(uint32_t band, const int8_t *hi, const int8_t *lo, const i
 }
 
 bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr, int32_t *lbound,
-                   uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, hipStream_t s) {
+                   uint32_t B, uint32_t S, uint32_t stride, uint32_t P, double step_sq, bool keep_sorted,
+                   hipStream_t s) {
   UpdateArgs a{pool, pool_cnt, pool_sorted, thr, lbound, B, S, stride, static_cast<int64_t>(P) * 65025, step_sq};
   const dim3 grid(B);
+  if (refine_cap(S) == BIG_SB && !keep_sorted) {
+    static const bool attr = (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_update_radix),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  static_cast<int>(UR_CHUNK * sizeof(uint64_t))),
+                              true);
+    (void)attr;
+    hipLaunchKernelGGL(k_update_radix, grid, dim3(256), UR_CHUNK * sizeof(uint64_t), s, a);
+    return true;
+  }
   // large pools by selection, then the rest (and any the selection left) by merging
   if (S <= 256) hipLaunchKernelGGL(k_update_sel, grid, dim3(256), 0, s, a);
   switch (refine_cap(S)) {
