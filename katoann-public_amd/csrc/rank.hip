@@ -2115,21 +2115,25 @@ __global__ __launch_bounds__(256) void k_refine_big(RefineArgs a) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t stmp[4];
   __shared__ uint32_t s_mb;
-  uint64_t *ex = lds, *buf = lds + BIG_SB;
+  // lists of C keys: the smallest power of two holding S (the launch sizes
+  // the LDS to three such lists)
+  uint32_t C = 64;
+  while (C < a.S) C <<= 1;
+  uint64_t *ex = lds, *buf = lds + C;
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint64_t *pq = a.pool + static_cast<size_t>(q) * a.stride;
   const uint32_t m = min(min(a.pool_cnt[static_cast<size_t>(q) * katoann_hip::kCntStride], a.stride), a.S);
   const uint64_t tq = a.thr[q];
-  for (uint32_t i = tid; i < BIG_SB; i += 256) {
+  for (uint32_t i = tid; i < C; i += 256) {
     const uint64_t k = i < m ? pq[i] : kNoKey;
     buf[i] = k <= tq ? k : kNoKey;
   }
   __syncthreads();
-  const uint32_t ns = block_count_sorted(buf, BIG_SB, stmp);
+  const uint32_t ns = block_count_sorted(buf, C, stmp);
   // the pool holds the survivors in no particular order (k_update_radix);
   // they arrive in descending key order, so sort them ascending first
-  const uint64_t *sv = block_sort_keys(buf, BIG_SB);
+  const uint64_t *sv = block_sort_keys(buf, C);
   if (a.surv)
     for (uint32_t i = tid; i < a.S; i += 256) a.surv[static_cast<size_t>(q) * a.S + i] = sv[i];
   if (a.nsurv && tid == 0) a.nsurv[q] = ns;
@@ -2138,7 +2142,7 @@ __global__ __launch_bounds__(256) void k_refine_big(RefineArgs a) {
   const int64_t qn = a.qnorm[q];
   for (uint32_t t0 = 16 * w; t0 < ns; t0 += 64) {
     const uint32_t s = t0 + (lane & 15);
-    const uint64_t key = sv[min(s, BIG_SB - 1)];
+    const uint64_t key = sv[min(s, C - 1)];
     const uint64_t e = exact_key16(a, key, s < ns, qh, ql, qn);
     if (lane < 16 && t0 + lane < ns) ex[t0 + lane] = e;
   }
@@ -2497,7 +2501,9 @@ bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_c
     case BIG_SB: {
       static const bool attr = (big_lds_attr(&k_refine_big), true);
       (void)attr;
-      hipLaunchKernelGGL(k_refine_big, dim3(B), dim3(256), BIG_LDS, s, a);
+      uint32_t C = 64;
+      while (C < S) C <<= 1;
+      hipLaunchKernelGGL(k_refine_big, dim3(B), dim3(256), 3 * C * sizeof(uint64_t), s, a);
       return true;
     }
     default: return false;
