@@ -1829,6 +1829,105 @@ __device__ inline uint64_t exact_key16(const RefineArgs &a, uint64_t key, bool s
   return make_key(fscale(qn + a.norm[myrow] - 2 * dot, a.step_sq), myid);
 }
 
+// Exact keys of one query's survivors ex[0, ns), one wave (the same MFMA
+// operands and int64 combine as exact_key16). On entry ex[i] = row << 32 | id
+// (row 0xffffffff: the id has no row), on return the exact key (kNoKey for no
+// row). The rows stream through as steps of 16 survivors x RB chunks of 64
+// bytes (both planes), RB dividing dc / 64, each step's gathers issued while
+// the step before is multiplied, so a wave keeps a step of loads in flight
+// from its first gather to its last; the survivors' norms travel with their
+// steps. qb: the query's high then low bytes in LDS (dc each), zb: 16 zero
+// bytes in LDS (the B operand's columns 2..15).
+constexpr int RF_AHEAD = 1;  // gather steps in flight while one is multiplied
+
+template <int RB>
+__device__ inline void exact_keys(const RefineArgs &a, uint64_t *ex, uint32_t ns, const int8_t *qb, const int8_t *zb,
+                                  int64_t qn) {
+  const int lane = lane_id(), kq = lane >> 4, col = lane & 15;
+  const uint32_t dc = a.dc, ncb = dc / (64 * RB), ng = (ns + 15) / 16, nsteps = ng * ncb;
+  if (nsteps == 0) return;
+  struct Step {
+    v4i xh[RB], xl[RB];
+    int64_t nrm;
+    uint32_t id;
+    bool ok;
+  };
+  // loads of step s (clamped: past the end it loads the last step again, so
+  // every iteration issues the same loads)
+  auto load = [&](Step &X, uint32_t s) {
+    s = min(s, nsteps - 1);
+    const uint32_t g = s / ncb, cb = s - g * ncb, i = 16 * g + col;
+    const uint64_t e = ex[min(i, ns - 1)];
+    const uint32_t row = static_cast<uint32_t>(e >> 32);
+    X.ok = i < ns && row != 0xffffffffu;
+    X.id = static_cast<uint32_t>(e);
+    const size_t base = static_cast<size_t>(X.ok ? row : 0u) * dc + cb * (64u * RB) + 16u * kq;
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      X.xh[u] = load16(a.hi + base + 64 * u);
+      X.xl[u] = load16(a.lo + base + 64 * u);
+    }
+    X.nrm = a.norm[X.ok ? row : 0u];
+  };
+  v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+  auto compute = [&](const Step &X, uint32_t s) {
+    const uint32_t g = s / ncb, cb = s - g * ncb;
+    if (cb == 0) ah = al = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const uint32_t kc = (cb * RB + u) * 64 + 16 * kq;
+      const int8_t *bp = col == 0 ? qb + kc : col == 1 ? qb + dc + kc : zb;
+      const v4i bq = *reinterpret_cast<const v4i *>(bp);
+      ah = mfma(X.xh[u], bq, ah);
+      al = mfma(X.xl[u], bq, al);
+    }
+    if (cb + 1 < ncb) return;
+    // survivor i = lane (< 16): C[i][0] in lane 16 (i / 4), element i % 4; C[i][1] one lane up
+    const int i = lane & 15, src = 16 * (i >> 2);
+    int32_t hh = 0, hl = 0, lh = 0, ll = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t a0 = __shfl(ah[r], src), a1 = __shfl(ah[r], src + 1);
+      const int32_t b0 = __shfl(al[r], src), b1 = __shfl(al[r], src + 1);
+      if ((i & 3) == r) {
+        hh = a0;
+        hl = a1;
+        lh = b0;
+        ll = b1;
+      }
+    }
+    // lane i < 16 holds survivor 16 g + i's id, norm and row flag (column i)
+    const int64_t dot =
+        static_cast<int64_t>(hh) * 65536 + (static_cast<int64_t>(hl) + lh) * 256 + static_cast<int64_t>(ll);
+    const uint64_t key = X.ok ? make_key(fscale(qn + X.nrm - 2 * dot, a.step_sq), X.id) : kNoKey;
+    if (lane < 16 && 16 * g + lane < ns) ex[16 * g + lane] = key;
+  };
+  Step X0, X1;
+  load(X0, 0);
+  if constexpr (RF_AHEAD == 1) {
+    for (uint32_t s = 0; s < nsteps; s += 2) {
+      load(X1, s + 1);
+      compute(X0, s);
+      if (s + 1 >= nsteps) break;
+      load(X0, s + 2);
+      compute(X1, s + 1);
+    }
+  } else {  // two steps ahead
+    Step X2;
+    load(X1, 1);
+    for (uint32_t s = 0; s < nsteps; s += 3) {
+      load(X2, s + 2);
+      compute(X0, s);
+      if (s + 1 >= nsteps) break;
+      load(X0, s + 3);
+      compute(X1, s + 1);
+      if (s + 2 >= nsteps) break;
+      load(X1, s + 4);
+      compute(X2, s + 2);
+    }
+  }
+}
+
 // Final survivors, exact distances, arrival rule, output (src/index.cpp:1046-1127):
 //  1. the S smallest keys of the query's pool (exactly the S smallest ranking
 //     keys of all its candidates);
@@ -1841,9 +1940,12 @@ __device__ inline uint64_t exact_key16(const RefineArgs &a, uint64_t key, bool s
 //     closed form, below);
 //  4. output ascending by (distance, id), padded with FLT_MAX / UINT32_MAX
 //     (src/index.cpp:384-398).
-template <int SC>
+constexpr uint32_t RF_MAXDC = 4096;  // code bytes per row at most (dimensions up to 4,096)
+
+template <int SC, int RB>
 __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   __shared__ uint64_t ex[4][SC];
+  __shared__ __attribute__((aligned(16))) int8_t qb[4][2 * RF_MAXDC + 16];
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x * 4 + w;
   if (q >= a.B) return;
@@ -1886,15 +1988,28 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
   }
   if (a.nsurv && lane == 0) a.nsurv[q] = ns;
 
-  const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
-  const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
-  const int64_t qn = a.qnorm[q];
-  for (uint32_t t0 = 0; t0 < ns; t0 += 16) {
-    const uint32_t s = t0 + (lane & 15);
-    const uint64_t key = top_get<SC>(sv, min(s, static_cast<uint32_t>(SC - 1)));
-    const uint64_t e = exact_key16(a, key, s < ns, qh, ql, qn);
-    if (lane < 16 && t0 + lane < ns) ex[w][t0 + lane] = e;
+  // the query's bytes and a zero block in LDS (the exact products' B operand)
+  {
+    const int8_t *qh = a.qhi + static_cast<size_t>(q) * a.dc;
+    const int8_t *ql = a.qlo + static_cast<size_t>(q) * a.dc;
+    for (uint32_t o = 16 * lane; o < a.dc; o += 1024) {
+      *reinterpret_cast<v4i *>(&qb[w][o]) = load16(qh + o);
+      *reinterpret_cast<v4i *>(&qb[w][a.dc + o]) = load16(ql + o);
+    }
+    if (lane == 0) *reinterpret_cast<v4i *>(&qb[w][2 * RF_MAXDC]) = v4i{0, 0, 0, 0};
   }
+  // each survivor's row << 32 | id, in survivor order
+#pragma unroll
+  for (int r = 0; r < SC / 64; ++r) {
+    const uint32_t i = lane + 64 * r;
+    if (i < ns) {
+      const uint32_t id = static_cast<uint32_t>(sv.e[r]);
+      const uint32_t row = id < a.npts ? a.id_row[id] : 0xffffffffu;
+      ex[w][i] = static_cast<uint64_t>(row) << 32 | id;
+    }
+  }
+  wave_fence();
+  exact_keys<RB>(a, ex[w], ns, qb[w], &qb[w][2 * RF_MAXDC], a.qnorm[q]);
   wave_fence();
 
   // The arrival rule in closed form. Survivors arrive in descending rank order
@@ -2492,12 +2607,22 @@ bool launch_refine(const uint64_t *pool, uint32_t stride, const uint32_t *pool_c
   RefineArgs a{pool, stride, pool_cnt, thr, B, S, K, hi, lo, norm, id_row, npts, dc, qhi, qlo, qnorm, step_sq,
                out_ids, out_dists, surv, nsurv};
   const dim3 grid((B + 3) / 4);
+  if (dc == 0 || dc % 64 || dc > RF_MAXDC) return false;
+  // chunks of 64 bytes per gather step: the largest of 8, 2, 1 dividing dc / 64
+  const uint32_t nk = dc / 64, rb = nk % 8 == 0 ? 8u : nk % 2 == 0 ? 2u : 1u;
+#define RF_CASE(sc)                                                                       \
+  case sc:                                                                                \
+    if (rb == 8) hipLaunchKernelGGL((k_refine<sc, 8>), grid, dim3(256), 0, s, a);         \
+    else if (rb == 2) hipLaunchKernelGGL((k_refine<sc, 2>), grid, dim3(256), 0, s, a);    \
+    else hipLaunchKernelGGL((k_refine<sc, 1>), grid, dim3(256), 0, s, a);                 \
+    return true;
   switch (refine_cap(S)) {
-    case 64: hipLaunchKernelGGL(k_refine<64>, grid, dim3(256), 0, s, a); return true;
-    case 128: hipLaunchKernelGGL(k_refine<128>, grid, dim3(256), 0, s, a); return true;
-    case 256: hipLaunchKernelGGL(k_refine<256>, grid, dim3(256), 0, s, a); return true;
-    case 512: hipLaunchKernelGGL(k_refine<512>, grid, dim3(256), 0, s, a); return true;
-    case 1024: hipLaunchKernelGGL(k_refine<1024>, grid, dim3(256), 0, s, a); return true;
+    RF_CASE(64)
+    RF_CASE(128)
+    RF_CASE(256)
+    RF_CASE(512)
+    RF_CASE(1024)
+#undef RF_CASE
     case BIG_SB: {
       static const bool attr = (big_lds_attr(&k_refine_big), true);
       (void)attr;
