@@ -875,7 +875,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     mark(1);
     if (ix.batched)
       katoann_hip::launch_coarse(w.qhi, w.qlo, w.qnorm, B, ix.cen_hi.as<int8_t>(), ix.cen_lo.as<int8_t>(),
-                                 ix.cen_norm.as<int64_t>(), nlist, dc, ix.step_sq, w.dist, s);
+                                 ix.cen_norm.as<int64_t>(), nlist, dc, ix.step_sq, w.dist, device_cus(), s);
     else
       katoann_hip::launch_coarse_flat(w.q, B, d, ix.cen_f.as<float>(), nlist, w.dist, s);
     mark(2);

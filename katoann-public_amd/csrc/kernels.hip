@@ -361,106 +361,197 @@ __device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
 
 // Per 64-byte chunk of the codes the four operand planes (queries and
 // centroids, high and low) are staged in LDS in MFMA fragment order (lane l's
-// 16 bytes at l * 16 of a 1 KB tile fragment, so every read is conflict-free);
-// the next chunk is loaded into registers while the current one is multiplied.
+// 16 bytes at l * 16 of a 1 KB tile fragment, so every read is conflict-free).
 constexpr int CO_T = 128;                    // queries and centroids per block tile
-constexpr int CO_STAGE = 4 * CO_T * 64;      // bytes of one chunk: 4 planes x 128 rows x 64
+constexpr int CO_PLANES = 4 * CO_T * 64;     // bytes of one chunk: 4 planes x 128 rows x 64
+constexpr int CO_NORMS = 2 * CO_T * 8;       // the tile's query and centroid norms (int64)
+constexpr int CO_STAGE = CO_PLANES + CO_NORMS;
+constexpr int CO_NS = 4;                     // LDS stages: copies run CO_NS - 1 chunks ahead
+constexpr size_t CO_SMEM = static_cast<size_t>(CO_NS) * CO_STAGE;
 
-// Block tile 128 queries x 128 centroids over 8 waves; wave (wq, wc) owns 64
-// queries x 32 centroids = 4 x 2 MFMA tiles with three accumulators each (hh,
-// hl + lh, ll), which leaves room for two waves per SIMD (one hides the
-// other's LDS reads and barrier waits). Output norms are loaded ahead of the
-// main loop.
+typedef __attribute__((address_space(3))) void co_lds_void;
+
+// Asynchronous copy of 16 / 4 bytes per lane from global memory into LDS: lane
+// l's bytes land at l * size past the wave-uniform LDS address.
+__device__ __forceinline__ void co_dma16(const void *g, void *l) {
+  __builtin_amdgcn_global_load_lds(g, (co_lds_void *)l, 16, 0, 0);
+}
+__device__ __forceinline__ void co_dma4(const void *g, void *l) {
+  __builtin_amdgcn_global_load_lds(g, (co_lds_void *)l, 4, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t co_lds_addr(const void *p) {
+  return static_cast<uint32_t>(reinterpret_cast<size_t>((const __attribute__((address_space(3))) char *)p));
+}
+
+// s_waitcnt immediate (gfx9 encoding) for the vector memory counter only: at
+// most n of the wave's vector memory loads in flight (loads complete in issue
+// order, so the older ones are done).
+constexpr int co_vm_imm(int n) { return (n & 0xf) | (((n >> 4) & 3) << 14) | (0x7 << 4) | (0xf << 8); }
+
+__device__ __forceinline__ void co_wait_vm(uint32_t n) {
+  switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(co_vm_imm(0)); break;
+    case 4: __builtin_amdgcn_s_waitcnt(co_vm_imm(4)); break;
+    case 5: __builtin_amdgcn_s_waitcnt(co_vm_imm(5)); break;
+    case 8: __builtin_amdgcn_s_waitcnt(co_vm_imm(8)); break;
+    case 9: __builtin_amdgcn_s_waitcnt(co_vm_imm(9)); break;
+    case 10: __builtin_amdgcn_s_waitcnt(co_vm_imm(10)); break;
+    default: __builtin_amdgcn_s_waitcnt(co_vm_imm(0)); break;
+  }
+}
+
+// LDS reads of one chunk's operands, written as asm that waits for its own
+// results: the compiler would otherwise make every LDS read wait for all the
+// copies into LDS in flight, the chunks ahead included. a: the wave's first
+// query fragment (high plane; the low plane 8 KB on), b: its first centroid
+// fragment (high plane; the low plane 8 KB on).
+__device__ __forceinline__ void co_read_hi(uint32_t a, uint32_t b, v4i (&ah)[4], v4i (&bh)[2], v4i (&bl)[2]) {
+  asm volatile(
+      "ds_read_b128 %0, %8\n\t"
+      "ds_read_b128 %1, %8 offset:1024\n\t"
+      "ds_read_b128 %2, %8 offset:2048\n\t"
+      "ds_read_b128 %3, %8 offset:3072\n\t"
+      "ds_read_b128 %4, %9\n\t"
+      "ds_read_b128 %5, %9 offset:1024\n\t"
+      "ds_read_b128 %6, %9 offset:8192\n\t"
+      "ds_read_b128 %7, %9 offset:9216\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(ah[0]), "=&v"(ah[1]), "=&v"(ah[2]), "=&v"(ah[3]), "=&v"(bh[0]), "=&v"(bh[1]), "=&v"(bl[0]),
+        "=&v"(bl[1])
+      : "v"(a), "v"(b));
+}
+__device__ __forceinline__ void co_read_lo(uint32_t a, v4i (&al)[4]) {
+  asm volatile(
+      "ds_read_b128 %0, %4 offset:8192\n\t"
+      "ds_read_b128 %1, %4 offset:9216\n\t"
+      "ds_read_b128 %2, %4 offset:10240\n\t"
+      "ds_read_b128 %3, %4 offset:11264\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(al[0]), "=&v"(al[1]), "=&v"(al[2]), "=&v"(al[3])
+      : "v"(a));
+}
+__device__ __forceinline__ int64_t co_read_i64(uint32_t a) {
+  int64_t x;
+  asm volatile(
+      "ds_read_b64 %0, %1\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(x)
+      : "v"(a));
+  return x;
+}
+
+// Coarse distances dist[q][c] = f(qn + cn - 2 dot(q, c)) of the 16-bit codes
+// (reference src/index.cpp:846-877), dot = 65536 hh + 256 (hl + lh) + ll from
+// four int8 MFMA products in int32 (|hh| <= 128 * 128 * Dc fits), combined in
+// int64.
+//
+// Persistent blocks of 8 waves, one per CU (the stages take 136 KB of LDS),
+// tiles of 128 queries x 128 centroids dealt round-robin over the blocks
+// (block b takes tiles b, b + grid, ...). Wave (wq, wc) owns 64 queries x 32
+// centroids = 4 x 2 MFMA tiles with three accumulators each (hh, hl + lh, ll);
+// two waves per SIMD, so one hides the other's LDS reads and barrier waits.
+// The four operand planes of every 64-byte chunk are copied straight from
+// HBM into an LDS ring of CO_NS stages (global_load_lds) CO_NS - 1 chunks
+// ahead, across tile boundaries, so the copies of the chunks ahead are in
+// flight while one is multiplied and the next tile's first chunks while a
+// tile's results are written. Wave w copies row block w (16 rows) of each
+// plane, and at a tile's first chunk also 64 dwords of the tile's norms: 4
+// copies per wave and chunk, 5 at a tile's first. Before a chunk, each wave
+// waits for its own copies of it (vector memory counter, those of the chunks
+// issued after it may stay in flight), then a barrier makes all eight waves'
+// copies visible and frees the stage the next copy overwrites.
 __global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ qhi, const int8_t *__restrict__ qlo,
                                                     const int64_t *__restrict__ qnorm, uint32_t B,
                                                     const int8_t *__restrict__ chi, const int8_t *__restrict__ clo,
                                                     const int64_t *__restrict__ cnorm, uint32_t nlist, uint32_t dc,
-                                                    double step_sq, float *__restrict__ dist) {
-  __shared__ __attribute__((aligned(16))) int8_t st[2][CO_STAGE];
-  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+                                                    double step_sq, float *__restrict__ dist, uint32_t nct,
+                                                    uint32_t ntiles) {
+  extern __shared__ __attribute__((aligned(16))) int8_t co_st[];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
   const int wq = w >> 2, wc = w & 3;  // 64 queries (wq) x 32 centroids (wc)
-  const uint32_t q0 = blockIdx.y * CO_T, c0 = blockIdx.x * CO_T;
-  // loader: thread t moves (row t / 4, slice t % 4) of each plane
-  const int lr = tid >> 2, ls = tid & 3;
-  const int8_t *src[4];
-  uint32_t dst[4];
-  {
-    const uint32_t r = static_cast<uint32_t>(lr);
-    const uint32_t qr = min(q0 + r, B - 1), cr = min(c0 + r, nlist - 1);
-    src[0] = qhi + static_cast<size_t>(qr) * dc + 16 * ls;
-    src[1] = qlo + static_cast<size_t>(qr) * dc + 16 * ls;
-    src[2] = chi + static_cast<size_t>(cr) * dc + 16 * ls;
-    src[3] = clo + static_cast<size_t>(cr) * dc + 16 * ls;
-    const uint32_t off = (r >> 4) * 1024 + (ls * 16 + (r & 15)) * 16;
-#pragma unroll
-    for (int pl = 0; pl < 4; ++pl) dst[pl] = pl * (CO_T * 64) + off;
-  }
-  v4i pre[4];
-  auto fetch = [&](uint32_t kc) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pre[i] = load16(src[i] + kc);
-  };
-  auto stash = [&](int b) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<v4i *>(&st[b][dst[i]]) = pre[i];
-  };
-  v4i hh[4][2], mid[4][2], ll[4][2];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
   const uint32_t nk = dc / 64;
-  fetch(0);
+  const uint32_t ntile_b = blockIdx.x < ntiles ? (ntiles - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  const uint32_t total = ntile_b * nk;  // chunks of this block
+  // copies of chunk g (chunk g % nk of the block's tile g / nk) into stage g % CO_NS
+  auto issue = [&](uint32_t g) {
+    const uint32_t kc = g % nk, t = blockIdx.x + (g / nk) * gridDim.x;
+    const uint32_t q0 = (t / nct) * CO_T, c0 = (t % nct) * CO_T;
+    const uint32_t r = 16u * w + (lane & 15);
+    const uint32_t qr = min(q0 + r, B - 1), cr = min(c0 + r, nlist - 1);
+    const uint32_t off = kc * 64 + 16 * (lane >> 4);
+    int8_t *S = co_st + (g % CO_NS) * CO_STAGE + w * 1024;
+    co_dma16(qhi + static_cast<size_t>(qr) * dc + off, S);
+    co_dma16(qlo + static_cast<size_t>(qr) * dc + off, S + CO_T * 64);
+    co_dma16(chi + static_cast<size_t>(cr) * dc + off, S + 2 * CO_T * 64);
+    co_dma16(clo + static_cast<size_t>(cr) * dc + off, S + 3 * CO_T * 64);
+    if (kc == 0) {  // dword j = 64 w + lane of [query norms | centroid norms]
+      const uint32_t j = 64u * w + lane, e = (j & 255u) >> 1;
+      const int64_t *src = j < 256 ? qnorm + min(q0 + e, B - 1) : cnorm + min(c0 + e, nlist - 1);
+      co_dma4(reinterpret_cast<const uint32_t *>(src) + (j & 1u), co_st + (g % CO_NS) * CO_STAGE + CO_PLANES + 256 * w);
+    }
+  };
+  const uint32_t pro = min(static_cast<uint32_t>(CO_NS - 1), total);
+  for (uint32_t g = 0; g < pro; ++g) issue(g);
+  uint32_t issued = pro;
+  v4i hh[4][2], mid[4][2], ll[4][2];
   int64_t qn[4][4], cn[2];
+  for (uint32_t g = 0; g < total; ++g) {
+    const uint32_t kc = g % nk;
+    uint32_t younger = 0;  // copies of the chunks issued after g
+    for (uint32_t h = g + 1; h < issued; ++h) younger += (h % nk == 0) ? 5u : 4u;
+    co_wait_vm(younger);
+    __builtin_amdgcn_s_barrier();
+    if (issued < total) issue(issued++);
+    const uint32_t S = co_lds_addr(co_st + (g % CO_NS) * CO_STAGE);
+    if (kc == 0) {
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) qn[a][r] = qnorm[min(q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r, B - 1)];
+        for (int b = 0; b < 2; ++b) hh[a][b] = mid[a][b] = ll[a][b] = v4i{0, 0, 0, 0};
 #pragma unroll
-  for (int t = 0; t < 2; ++t) cn[t] = cnorm[min(c0 + 32 * wc + 16 * t + (lane & 15), nlist - 1)];
-  stash(0);
-  __syncthreads();
-  for (uint32_t k = 0; k < nk; ++k) {
-    const int b = k & 1;
-    if (k + 1 < nk) fetch((k + 1) * 64);
-    const int8_t *S = st[b];
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          qn[a][r] = co_read_i64(S + CO_PLANES + 8 * (64 * wq + 16 * a + 4 * (lane >> 4) + r));
+#pragma unroll
+      for (int t = 0; t < 2; ++t) cn[t] = co_read_i64(S + CO_PLANES + 1024 + 8 * (32 * wc + 16 * t + (lane & 15)));
+    }
     v4i bh[2], bl[2], ah[4], al[4];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      bh[t] = *reinterpret_cast<const v4i *>(S + 2 * CO_T * 64 + (2 * wc + t) * 1024 + lane * 16);
-      bl[t] = *reinterpret_cast<const v4i *>(S + 3 * CO_T * 64 + (2 * wc + t) * 1024 + lane * 16);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      ah[a] = *reinterpret_cast<const v4i *>(S + (4 * wq + a) * 1024 + lane * 16);
-      al[a] = *reinterpret_cast<const v4i *>(S + CO_T * 64 + (4 * wq + a) * 1024 + lane * 16);
-    }
+    co_read_hi(S + (4 * wq) * 1024 + lane * 16, S + 2 * CO_T * 64 + (2 * wc) * 1024 + lane * 16, ah, bh, bl);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         hh[a][t] = mfma(ah[a], bh[t], hh[a][t]);
         mid[a][t] = mfma(ah[a], bl[t], mid[a][t]);
-        mid[a][t] = mfma(al[a], bh[t], mid[a][t]);
-        ll[a][t] = mfma(al[a], bl[t], ll[a][t]);
       }
-    if (k + 1 < nk) {
-      stash(b ^ 1);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const uint32_t c = c0 + 32 * wc + 16 * t + (lane & 15);
+    co_read_lo(S + (4 * wq) * 1024 + lane * 16, al);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t q = q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r;
-        const int64_t dot = static_cast<int64_t>(hh[a][t][r]) * 65536 + static_cast<int64_t>(mid[a][t][r]) * 256 +
-                            static_cast<int64_t>(ll[a][t][r]);
-        if (q < B && c < nlist) dist[static_cast<size_t>(q) * nlist + c] = fscale(qn[a][r] + cn[t] - 2 * dot, step_sq);
+      for (int t = 0; t < 2; ++t) {
+        mid[a][t] = mfma(al[a], bh[t], mid[a][t]);
+        ll[a][t] = mfma(al[a], bl[t], ll[a][t]);
       }
+    if (kc == nk - 1) {
+      const uint32_t tile = blockIdx.x + (g / nk) * gridDim.x;
+      const uint32_t q0 = (tile / nct) * CO_T, c0 = (tile % nct) * CO_T;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const uint32_t c = c0 + 32 * wc + 16 * t + (lane & 15);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t q = q0 + 64 * wq + 16 * a + 4 * (lane >> 4) + r;
+            const int64_t dot = static_cast<int64_t>(hh[a][t][r]) * 65536 +
+                                static_cast<int64_t>(mid[a][t][r]) * 256 + static_cast<int64_t>(ll[a][t][r]);
+            if (q < B && c < nlist)
+              dist[static_cast<size_t>(q) * nlist + c] = fscale(qn[a][r] + cn[t] - 2 * dot, step_sq);
+          }
+      }
+    }
   }
 }
 
@@ -1234,10 +1325,17 @@ void launch_minmax(const float *y, uint64_t n, float *part, uint32_t nblocks, hi
 
 void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, uint32_t B, const int8_t *chi,
                    const int8_t *clo, const int64_t *cnorm, uint32_t nlist, uint32_t dc, double step_sq,
-                   float *dist, hipStream_t s) {
-  dim3 grid((nlist + CO_T - 1) / CO_T, (B + CO_T - 1) / CO_T);
-  hipLaunchKernelGGL(k_coarse8, grid, dim3(512), 0, s, qhi, qlo, qnorm, B, chi, clo, cnorm, nlist, dc, step_sq,
-                     dist);
+                   float *dist, int cus, hipStream_t s) {
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse8),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(CO_SMEM));
+    return true;
+  }();
+  (void)attr;
+  const uint32_t nct = (nlist + CO_T - 1) / CO_T, ntiles = nct * ((B + CO_T - 1) / CO_T);
+  const uint32_t grid = min(ntiles, static_cast<uint32_t>(cus));
+  hipLaunchKernelGGL(k_coarse8, dim3(grid), dim3(512), CO_SMEM, s, qhi, qlo, qnorm, B, chi, clo, cnorm, nlist, dc,
+                     step_sq, dist, nct, ntiles);
 }
 
 size_t select_smem(uint32_t nlist, uint32_t np) {

@@ -26,7 +26,7 @@ void launch_quantize(const float *Y, uint32_t n, uint32_t d, uint32_t dc, uint32
 void launch_minmax(const float *y, uint64_t n, float *part, uint32_t nblocks, hipStream_t s);
 void launch_coarse(const int8_t *qhi, const int8_t *qlo, const int64_t *qnorm, uint32_t B, const int8_t *chi,
                    const int8_t *clo, const int64_t *cnorm, uint32_t nlist, uint32_t dc, double step_sq,
-                   float *dist, hipStream_t s);
+                   float *dist, int cus, hipStream_t s);
 // Per-query fp32 coarse distances (LoadOptions::batched_coarse = false):
 // dist[q][c] = l2_fp32(unrotated query q, float centroid c).
 void launch_coarse_flat(const float *Q, uint32_t B, uint32_t d, const float *cen, uint32_t nlist, float *dist,

@@ -879,6 +879,14 @@ struct QFrag {
   uint32_t q;
 };
 
+// Item descriptor i through the scalar cache (wave-uniform i): the table was
+// written by an earlier launch and is read-only here.
+__device__ __forceinline__ uint4 item_desc(const uint4 *items, uint32_t i) {
+  typedef const __attribute__((address_space(4))) uint32_t *cptr;
+  const cptr q = (cptr)(reinterpret_cast<uintptr_t>(items + i));
+  return uint4{q[0], q[1], q[2], q[3]};
+}
+
 // One item of the wave form (rows in registers, queries streamed).
 template <int KC, int T, int NB>
 __device__ __forceinline__ void wave_item(const uint4 dv, const WaveArgs &a, RunStage &rs, uint32_t &n_kept) {
@@ -975,10 +983,18 @@ __global__ __launch_bounds__(256, (T * KC > 32 ? 1 : 2)) void k_rank_wave(WaveAr
   RunStage rs{s_key[w], s_run[w], s_rq[w], s_rfirst[w], s_rbase[w], 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
-  // items dealt round-robin over the grid's waves: a shared queue counter
-  // serialises every taker on one address
+  // items dealt round-robin over the grid's waves (a shared queue counter
+  // serialises every taker on one address); each wave reads its next item's
+  // descriptor through the scalar cache while it ranks the current one, so
+  // no descriptor round trip precedes an item's row loads
   const uint32_t nw = gridDim.x * 4;
-  for (uint32_t it = ib + blockIdx.x * 4 + w; it < ie; it += nw) wave_item<KC, T, NB>(a.items[it], a, rs, n_kept);
+  uint32_t it = ib + blockIdx.x * 4 + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(w));
+  uint4 dn = it < ie ? item_desc(a.items, it) : uint4{0, 0, 0, 0};
+  for (; it < ie; it += nw) {
+    const uint4 d = dn;
+    if (it + nw < ie) dn = item_desc(a.items, it + nw);
+    wave_item<KC, T, NB>(d, a, rs, n_kept);
+  }
   if (rs.n) runs_flush(rs, a);
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
 }
