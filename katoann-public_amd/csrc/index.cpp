@@ -784,7 +784,8 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.item_off = (z.nslots + 1ull) * 4;
   z.pool_cnt = B * 4 * katoann_hip::kCntStride;
   z.thr = B * 8;
-  z.state = z.nstate * 4ull;
+  // the state, then the ranking items' counts and cursors per (band, cost class)
+  z.state = (z.nstate + katoann_hip::kItemHistWords) * 4ull;
   z.items = static_cast<size_t>(z.item_cap) * 16;
   z.out = B * z.K * 4;
   z.surv = (x && x->survivors) ? B * z.S * 8 : 0;
@@ -900,10 +901,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       return;
     }
     katoann_hip::launch_plan(w.dist, w.probes, w.probe_d, npq, caller_npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
-                             qc, rt, z.qsq, z.qsrt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals, w.pool_cnt,
-                             w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, s);
+                             qc, rt, z.qsq, z.qsrt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals,
+                             w.pool_cnt, w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, s);
     katoann_hip::launch_items(w.item_off, nslots, nlist, ix.list_off_d.as<uint64_t>(), w.gcnt, w.qoff, qc, rt, z.qsq,
-                              z.qsrt, item_cap, z.seed, w.items, s);
+                              z.qsrt, item_cap, z.seed, w.state + nstate, w.items, s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();

@@ -1140,8 +1140,12 @@ __device__ inline uint64_t block_exscan(uint64_t v, uint64_t *tmp, uint64_t *tot
 __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t qtiles, uint32_t nslots,
                                                       uint32_t nlist, const uint64_t *__restrict__ list_off,
                                                       uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                                                      uint32_t seed, uint32_t *gcnt, uint32_t *nit, uint32_t *bsum) {
+                                                      uint32_t seed, uint32_t *gcnt, uint32_t *nit, uint32_t *bsum,
+                                                      uint32_t *ihist) {
   __shared__ uint64_t tmp[17];
+  __shared__ uint32_t hist[katoann_hip::kPlanBands * katoann_hip::kItemClasses];
+  for (uint32_t i = threadIdx.x; i < katoann_hip::kPlanBands * katoann_hip::kItemClasses; i += 1024) hist[i] = 0;
+  __syncthreads();
   const uint32_t sl = blockIdx.x * 1024 + threadIdx.x;
   uint32_t g = 0, it = 0;
   if (sl < nslots) {
@@ -1160,7 +1164,20 @@ __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t q
                                         : ((g + qc - 1) / qc) * ((n + rt - 1) / rt);
     gcnt[sl] = g;
     nit[sl] = it;
+    // the slot's items per cost class (k_items places them by class): all
+    // with g queries (query-stationary), or chunks of qc and a last one
+    uint32_t *hb = hist + (sl / nlist) * katoann_hip::kItemClasses;
+    if (it && katoann_hip::slot_qs(g, qsq)) {
+      atomicAdd(&hb[katoann_hip::item_class(g)], it);
+    } else if (it) {
+      const uint32_t nch = (g + qc - 1) / qc, nt = (n + rt - 1) / rt;
+      if (nch > 1) atomicAdd(&hb[katoann_hip::item_class(qc)], (nch - 1) * nt);
+      atomicAdd(&hb[katoann_hip::item_class(g - (nch - 1) * qc)], nt);
+    }
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < katoann_hip::kPlanBands * katoann_hip::kItemClasses; i += 1024)
+    if (hist[i]) atomicAdd(&ihist[i], hist[i]);
   uint64_t tg = 0, ti = 0;
   (void)block_exscan(g, tmp, &tg);
   (void)block_exscan(it, tmp, &ti);
@@ -1419,7 +1436,9 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
   uint32_t *seed_r0 = bsum + 2 * nblk;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
-  hipLaunchKernelGGL(k_fill32, dim3((nstate + 255) / 256), dim3(256), 0, s, state, nstate, 0u);
+  // the state and, after it, the per (band, class) item counts and cursors
+  const uint32_t nclear = nstate + katoann_hip::kItemHistWords;
+  hipLaunchKernelGGL(k_fill32, dim3((nclear + 255) / 256), dim3(256), 0, s, state, nclear, 0u);
   if (npq && check_npq)
     hipLaunchKernelGGL(k_check_npq, dim3((B + 255) / 256), dim3(256), 0, s, const_cast<uint32_t *>(npq), B, np, state);
   if (seed)
@@ -1433,7 +1452,7 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                        list_off, part, nullptr, nullptr);
     // (one partial count per slot: the counts become the slots' cursors, at 0)
     hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
-                       gcnt, nit, bsum);
+                       gcnt, nit, bsum, state + nstate);
     hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
     hipLaunchKernelGGL(k_inv_probe<true>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
                        list_off, part, qoff, qlist);
@@ -1443,7 +1462,7 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                          nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
-                     gcnt, nit, bsum);
+                     gcnt, nit, bsum, state + nstate);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
   inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                         qoff, qlist);

@@ -17,6 +17,21 @@ constexpr uint32_t kCntStride = 32;
 // of qc queries and rt rows as usual.
 constexpr uint32_t kQsItem = 0x80000000u;
 __host__ __device__ inline bool slot_qs(uint32_t g, uint32_t qsq) { return qsq && g <= qsq; }
+// Cost classes of the ranking items. A ranking kernel deals a band's items to
+// its waves in item order, and an item's time grows with the query fragments
+// it streams (16 queries each): dealt in list order, some waves drew far more
+// fragments than others and the band waited for them. Each band's items are
+// therefore laid out by class, most fragments first (> 64, 49-64, 33-48,
+// 25-32, 17-24, 13-16, 9-12, 7-8, 5-6, 3-4, 2, 1), and dealt in rounds.
+constexpr uint32_t kItemClasses = 12;
+constexpr uint32_t kPlanBands = 16;  // bands of one plan at most
+// Words after the plan's state: per (band, class) item counts, then cursors.
+constexpr uint32_t kItemHistWords = 2 * kPlanBands * kItemClasses;
+__host__ __device__ inline uint32_t item_class(uint32_t nq) {
+  const uint32_t nf = (nq + 15) / 16;
+  return nf > 64 ? 0u : nf > 48 ? 1u : nf > 32 ? 2u : nf > 24 ? 3u : nf > 16 ? 4u : nf > 12 ? 5u
+       : nf > 8 ? 6u : nf > 6 ? 7u : nf > 4 ? 8u : nf > 2 ? 9u : nf > 1 ? 10u : 11u;
+}
 
 void launch_normalize(float *x, uint32_t n, uint32_t d, hipStream_t s);
 void launch_rotate(const float *X, uint32_t n, uint32_t d, const float *R, float *Y, hipStream_t s);
@@ -79,9 +94,12 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
 void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
+// Item descriptors, each band's in cost-class order; ihist: the per (band,
+// class) item counts the plan left after its state (nstate words), and their
+// cursors.
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
                   const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                  uint32_t cap, uint32_t seed, uint4 *items, hipStream_t s);
+                  uint32_t cap, uint32_t seed, uint32_t *ihist, uint4 *items, hipStream_t s);
 // Rows [first, end) of list c's n rows that slot band `band` ranks.
 __host__ __device__ inline void slot_rows(uint32_t band, uint64_t n, uint32_t seed, uint64_t &first, uint64_t &end) {
   first = 0;

@@ -84,47 +84,98 @@ struct RankShape {
   static constexpr size_t SMEM = 2 * BUF + 4 * CAP * 10;
 };
 
-// Descriptor of every ranking item: thread per item, binary search of its
-// (band, list) slot in item_off.
-__global__ void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots, uint32_t nlist,
-                        const uint64_t *__restrict__ list_off, const uint32_t *__restrict__ gcnt,
-                        const uint32_t *__restrict__ qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                        uint32_t cap, uint32_t seed, uint4 *items) {
+// Descriptor of every ranking item: thread per item (slot order: binary search
+// of its (band, list) slot in item_off), placed within its band by cost class
+// (katoann_hip::item_class of its query count; ihist: the per (band, class)
+// counts, then cursors). A block's items of one (band, class) take one run of
+// positions (one atomic on the cursor), in item order within each wave, so a
+// row tile's query chunks stay neighbours and are dealt to neighbouring waves
+// of one round: they read the tile's rows from HBM about once per L2.
+__global__ __launch_bounds__(256) void k_items(const uint32_t *__restrict__ item_off, uint32_t nslots,
+                                               uint32_t nlist, const uint64_t *__restrict__ list_off,
+                                               const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ qoff,
+                                               uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt, uint32_t cap,
+                                               uint32_t seed, uint32_t *ihist, uint4 *items) {
+  constexpr uint32_t NC = katoann_hip::kItemClasses, NH = katoann_hip::kPlanBands * NC;
+  __shared__ uint32_t base[NH];   // first position of (band, class) in the items, then this block's run
+  __shared__ uint32_t cnt[NH];    // this block's items per (band, class)
+  __shared__ uint32_t wcnt[4][NH];
+  const uint32_t total = min(item_off[nslots], cap);
+  if (blockIdx.x * 256 >= total) return;
   const uint32_t it = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t total = item_off[nslots];
-  if (it >= total || it >= cap) return;
-  uint32_t lo = 0, hi = nslots;  // item_off[lo] <= it < item_off[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (item_off[mid] <= it) lo = mid;
-    else hi = mid;
+  const bool valid = it < total;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const uint32_t nb = (nslots + nlist - 1) / nlist;
+  for (uint32_t i = threadIdx.x; i < NH; i += 256) {
+    cnt[i] = 0;
+    wcnt[0][i] = wcnt[1][i] = wcnt[2][i] = wcnt[3][i] = 0;
   }
-  const uint32_t slot = lo, c = slot % nlist;
-  uint64_t first, end;
-  katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
-  const uint64_t n = end - first;
-  const uint32_t local0 = it - item_off[slot];
-  if (katoann_hip::slot_qs(gcnt[slot], qsq)) {  // all the slot's queries, rows qsrt at a time
-    uint4 d;
-    d.x = qoff[slot];
-    d.y = gcnt[slot] | katoann_hip::kQsItem;
-    d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(local0) * qsrt);
-    d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(local0 + 1) * qsrt));
-    items[it] = d;
-    return;
+  if (threadIdx.x < nb) {  // band b's classes start at its first item, most fragments first
+    uint32_t at = item_off[threadIdx.x * nlist];
+    for (uint32_t c = 0; c < NC; ++c) {
+      base[threadIdx.x * NC + c] = at;
+      at += ihist[threadIdx.x * NC + c];
+    }
   }
-  // the query chunks of one row tile are consecutive items: dealt to
-  // neighbouring waves (one CU, then the next block's), they read the tile's
-  // rows from HBM about once per L2 instead of once per chunk
-  const uint32_t nch = (gcnt[slot] + qc - 1) / qc;
-  const uint32_t local = it - item_off[slot];
-  const uint32_t qch = local % nch, rtile = local / nch;
-  uint4 d;
-  d.x = qoff[slot] + qch * qc;
-  d.y = min(qc, gcnt[slot] - qch * qc);
-  d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(rtile) * rt);
-  d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(rtile + 1) * rt));
-  items[it] = d;
+  uint4 d{0, 0, 0, 0};
+  uint32_t key = 0;  // band * NC + class
+  if (valid) {
+    uint32_t lo = 0, hi = nslots;  // item_off[lo] <= it < item_off[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (item_off[mid] <= it) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t slot = lo, c = slot % nlist;
+    uint64_t first, end;
+    katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
+    const uint64_t n = end - first;
+    const uint32_t local = it - item_off[slot];
+    if (katoann_hip::slot_qs(gcnt[slot], qsq)) {  // all the slot's queries, rows qsrt at a time
+      d.x = qoff[slot];
+      d.y = gcnt[slot] | katoann_hip::kQsItem;
+      d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(local) * qsrt);
+      d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(local + 1) * qsrt));
+      key = (slot / nlist) * NC + katoann_hip::item_class(gcnt[slot]);
+    } else {  // the query chunks of one row tile are consecutive items
+      const uint32_t nch = (gcnt[slot] + qc - 1) / qc;
+      const uint32_t qch = local % nch, rtile = local / nch;
+      d.x = qoff[slot] + qch * qc;
+      d.y = min(qc, gcnt[slot] - qch * qc);
+      d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(rtile) * rt);
+      d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(rtile + 1) * rt));
+      key = (slot / nlist) * NC + katoann_hip::item_class(d.y);
+    }
+  }
+  __syncthreads();
+  // rank within the wave among its items of the same key (lane order), the
+  // wave's count per key, then the waves' offsets and the block's run
+  uint32_t rank = 0;
+  {
+    uint64_t todo = ballot(valid);
+    while (todo) {
+      const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(key), __ffsll(static_cast<long long>(todo)) - 1));
+      const uint64_t m = ballot(valid && key == k) & todo;
+      if (valid && key == k) rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+      if (lane == 0) wcnt[w][k] = static_cast<uint32_t>(__popcll(m));
+      todo &= ~m;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < NH; i += 256) {
+    const uint32_t c0 = wcnt[0][i], c1 = wcnt[1][i], c2 = wcnt[2][i], c3 = wcnt[3][i];
+    wcnt[0][i] = 0;
+    wcnt[1][i] = c0;
+    wcnt[2][i] = c0 + c1;
+    wcnt[3][i] = c0 + c1 + c2;
+    cnt[i] = c0 + c1 + c2 + c3;
+    if (cnt[i]) base[i] += atomicAdd(&ihist[NH + i], cnt[i]);
+  }
+  __syncthreads();
+  if (valid) {
+    const uint32_t pos = base[key] + wcnt[w][key] + rank;
+    if (pos < cap) items[pos] = d;
+  }
 }
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -983,17 +1034,23 @@ __global__ __launch_bounds__(256, (T * KC > 32 ? 1 : 2)) void k_rank_wave(WaveAr
   RunStage rs{s_key[w], s_run[w], s_rq[w], s_rfirst[w], s_rbase[w], 0u, 0u};
   const uint32_t ib = a.item_off[a.band * a.nlist], ie = a.item_off[(a.band + 1) * a.nlist];
   uint32_t n_kept = 0;
-  // items dealt round-robin over the grid's waves (a shared queue counter
-  // serialises every taker on one address); each wave reads its next item's
-  // descriptor through the scalar cache while it ranks the current one, so
-  // no descriptor round trip precedes an item's row loads
-  const uint32_t nw = gridDim.x * 4;
-  uint32_t it = ib + blockIdx.x * 4 + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(w));
+  // The band's items come in cost order (most query fragments first) and are
+  // dealt to the grid's waves in rounds of one item per wave, alternately
+  // forward and backward (round r: item r nw + j to wave j, or to wave nw - 1 -
+  // j), so every wave draws about the same work (a shared queue counter
+  // serialises every taker on one address). Each wave reads its next item's
+  // descriptor through the scalar cache while it ranks the current one.
+  const uint32_t nw = gridDim.x * 4, wg = blockIdx.x * 4 + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(w));
+  auto item_at = [&](uint32_t r) { return ib + r * nw + ((r & 1u) ? nw - 1u - wg : wg); };
+  uint32_t r = 0, it = item_at(0);
   uint4 dn = it < ie ? item_desc(a.items, it) : uint4{0, 0, 0, 0};
-  for (; it < ie; it += nw) {
+  while (it < ie) {
     const uint4 d = dn;
-    if (it + nw < ie) dn = item_desc(a.items, it + nw);
+    const uint32_t nx = item_at(r + 1);
+    if (nx < ie) dn = item_desc(a.items, nx);
     wave_item<KC, T, NB>(d, a, rs, n_kept);
+    ++r;
+    it = nx;
   }
   if (rs.n) runs_flush(rs, a);
   if (lane == 0 && n_kept) atomicAdd(&a.state[1], n_kept);
@@ -1976,11 +2033,11 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
     // updated after the last band: it holds the S sorted survivors of the
     // earlier bands, then the last band's keys)
     uint64_t tq = a.thr[q];
-    for (uint32_t b = 0; b < m; b += 64) {
+    auto chunk = [&](uint32_t b) {
       uint64_t k = b + lane < m ? a.pool[base + b + lane] : kNoKey;
       if (k > tq) k = kNoKey;
       const uint64_t mk = ballot(k != kNoKey);
-      if (!mk) continue;
+      if (!mk) return;
       if (__popcll(mk) <= 4) {
         uint64_t mm = mk;
         while (mm) {
@@ -1992,7 +2049,15 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
         top_merge<SC>(sv, k, a.S);
       }
       tq = umin64(tq, top_get<SC>(sv, a.S - 1));
-    }
+    };
+    // The chunks holding the earlier survivors first, then the last band's
+    // keys in four interleaved passes (every fourth chunk): the filter
+    // tightens on a sample spread over the whole band, whatever order the
+    // ranking kernel appended its keys in.
+    const uint32_t nch = (m + 63) / 64, c0 = min(nch, (a.S + 63) / 64);
+    for (uint32_t c = 0; c < c0; ++c) chunk(64 * c);
+    for (uint32_t p = 0; p < 4; ++p)
+      for (uint32_t c = c0 + p; c < nch; c += 4) chunk(64 * c);
   }
   const uint32_t ns = top_count<SC>(sv);
   if (a.surv) {
@@ -2356,9 +2421,9 @@ uint32_t rank_queries_per_item(uint32_t P) {
 
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
                   const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                  uint32_t cap, uint32_t seed, uint4 *items, hipStream_t s) {
+                  uint32_t cap, uint32_t seed, uint32_t *ihist, uint4 *items, hipStream_t s) {
   hipLaunchKernelGGL(k_items, dim3((cap + 255) / 256), dim3(256), 0, s, item_off, nslots, nlist, list_off, gcnt, qoff,
-                     qc, rt, qsq, qsrt, cap, seed, items);
+                     qc, rt, qsq, qsrt, cap, seed, ihist, items);
 }
 
 template <int KC>
