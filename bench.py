@@ -123,11 +123,12 @@ def run_workload(name, args, dev, headline):
     if args.profile_point:  # one point, with the tile pinned (profiler runs)
         grid = [tuple(int(x) for x in args.profile_point.split(":"))]
         if args.tile:
-            kern, var, qc, shift, seed = (int(x) for x in args.tile.split(","))
+            kern, var, qc, shift, seed, join = (tuple(int(x) for x in args.tile.split(",")) + (0,))[:6]
             ix.set_autotune(False)
             ix.set_rank_kernel(kern, var, qc)
             ix.set_band_factor(shift)
             ix.set_seed_rows(seed)
+            ix.set_seed_join(join)
     sweep, outs = sweep_points(ix, queries, gt, grid, sizes, pd, args)
     b99, b999 = best(sweep, 0.99), best(sweep, 0.999)
     head = b99 or max(sweep, key=lambda r: r["recall"])
@@ -279,12 +280,14 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
     prof = None if args.profile_point else profiled(name, head)
     tile = head["tile"]
     if prof:
-        kern, var, qc, shift, seed = prof["tile"]
+        kern, var, qc, shift, seed, join = (tuple(prof["tile"]) + (0,))[:6]
         ix.set_autotune(False)
         ix.set_rank_kernel(kern, var, qc)
         ix.set_band_factor(shift)
         ix.set_seed_rows(seed)
-        tile = dict(kernel=kern, variant=var, queries_per_item=qc, log2_band_factor=shift, seed_rows=seed)
+        ix.set_seed_join(join)
+        tile = dict(kernel=kern, variant=var, queries_per_item=qc, log2_band_factor=shift, seed_rows=seed,
+                    seed_join=join)
         # a pinned tile leaves the tuned ones for other shapes untouched: the
         # tuner's table is per shape, and this one is now measured with the pin
     import torch
@@ -305,6 +308,7 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
         ix.set_rank_kernel(1, 0, 1024)
         ix.set_band_factor(5)
         ix.set_seed_rows(1024)
+        ix.set_seed_join(0)
         ix.set_autotune(True)
     rank_ms = st["rank"]
     byts, ops = head["rank_bytes"], head["rank_ops"]
@@ -660,7 +664,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered queries/s as a fraction of batch QPS")
     ap.add_argument("--profile-point", default="",
                     help="NPROBE:S: measure only this point of the headline workload (profiler runs)")
-    ap.add_argument("--tile", default="", help="kernel,variant,queries_per_item,log2_band_factor,seed_rows to pin "
+    ap.add_argument("--tile", default="", help="kernel,variant,queries_per_item,log2_band_factor,seed_rows[,seed_join] to pin "
                                                "with --profile-point")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, three device copies of this many MiB (counter calibration)")

@@ -254,6 +254,13 @@ int katoann_set_band_factor(katoann_index *h, uint32_t log2_factor);
  * query thus has a threshold before its wider bands. 0 = bands from rank 0.
  * Default 1024. Results do not depend on it. */
 int katoann_set_seed_rows(katoann_index *h, uint32_t rows);
+/* Joined seed remainder (no reference counterpart): with on != 0, a seed band
+ * and nprobe <= nlist / 16, the rest of each query's seed lists (rows past R0)
+ * is ranked in the first probe-rank band with that band's queries of the same
+ * list, on the thresholds the seed band produced, instead of in a band of its
+ * own: those rows are read once and one pool update goes. Default off.
+ * Results do not depend on it. */
+int katoann_set_seed_join(katoann_index *h, int32_t on);
 /* Survivor pool room per query beyond S for one band's kept keys (no
  * reference counterpart): 0 (default) sizes it per search. A band that keeps
  * more runs the batch again with 4x the room; later searches of the shape
@@ -286,12 +293,12 @@ int katoann_set_graphs(katoann_index *h, int32_t on);
 /* Tile autotuning (no reference counterpart): with it on, the first search of
  * each shape (ranking width, nlist, batch rounded up to a power of two,
  * nprobe, max(survivors, k)) times the candidate ranking tiles (kernel, row
- * tile, queries per item, band factor, seed rows) on its own inputs and keeps the fastest
+ * tile, queries per item, band factor, seed rows, joined seed remainder) on its own inputs and keeps the fastest
  * for that shape; results are the same whatever the tile. Default off. */
 int katoann_set_autotune(katoann_index *h, int32_t on);
 /* The tile a search of this shape runs with: kernel, variant, queries per
- * item, log2 band factor, seed rows (out: 5 entries). */
-int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out5);
+ * item, log2 band factor, seed rows, joined seed remainder (out: 6 entries). */
+int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_search_params *p, uint32_t *out6);
 
 /* Kernel-level distances (reference kernels.h:34-52) on the GPU, host memory in
  * and out, row-major planes (m x padded_dim and n x padded_dim bytes), norms of

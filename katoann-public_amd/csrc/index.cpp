@@ -112,7 +112,8 @@ struct SearchCtx {
 // items with a query ring), row-tile variant, queries per item, log2 band factor.
 struct TileConfig {
   uint32_t kernel, variant, qc, shift;
-  uint32_t seed;  // rows of the seed band (0: geometric bands from rank 0)
+  uint32_t seed;      // rows of the seed band (0: geometric bands from rank 0)
+  uint32_t join = 0;  // the seed remainder ranked in the first probe-rank band (kernels.hpp)
 };
 using TileKey = std::array<uint32_t, 6>;  // P, nlist, batch rounded up to 2^k, nprobe, S, ranking
 
@@ -128,6 +129,7 @@ struct katoann_index {
   uint32_t rank_rows = 1024;
   uint32_t band_shift = 5;     // probe-rank bands grow by 2^band_shift
   uint32_t seed_rows = 1024;  // rows of the seed band (0: none)
+  uint32_t seed_join = 0;     // seed remainder joined to the first probe-rank band
   uint32_t pool_allowance = 0;  // pool slots per query beyond S (0: sized per search)
   uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 1024;
   std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
@@ -560,7 +562,7 @@ void build_device_index(katoann_index &ix, uint32_t dim, uint32_t nlist, uint32_
 // Validated shape of one search and the bytes of every per-search buffer, so
 // that the search and the admission estimate of a serving loop agree.
 struct SearchPlan {
-  uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, qc, nslots, nbands, nstate, rt, item_cap, variant;
+  uint32_t B, d, dc, np, K, S, nlist, P, shift, seed, join, qc, nslots, nbands, nstate, rt, item_cap, variant;
   bool prefix, full, bf16, wave, ring, qs, hyb, gemm;
   uint32_t qsq, qsrt;  // hybrid items: slots of at most qsq queries as query-stationary items of qsrt rows
   uint64_t max_rows, allowance;
@@ -670,7 +672,7 @@ TileConfig tile_for(const katoann_index &ix, uint32_t nq, const katoann_search_p
     auto it = mix.tuned.find(tile_key(ix, nq, p));
     if (it != mix.tuned.end()) return it->second;
   }
-  return TileConfig{ix.rank_kernel, ix.wave_variant, ix.wave_qc, ix.band_shift, ix.seed_rows};
+  return TileConfig{ix.rank_kernel, ix.wave_variant, ix.wave_qc, ix.band_shift, ix.seed_rows, ix.seed_join};
 }
 
 SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_search_params &p,
@@ -732,6 +734,12 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
   z.nslots = katoann_hip::plan_slots(z.nlist, z.np, z.shift, z.seed);
   z.nbands = katoann_hip::plan_bands(z.np, z.shift, z.seed);
   if (z.nbands > kMaxBands) throw std::invalid_argument("too many probe-rank bands");
+  // the joined seed remainder needs the probe-list inversion and a first
+  // probe-rank band (bands 0, 1, 2)
+  z.join = tc.join && z.seed && katoann_hip::inv_from_probes(z.nlist, z.np) && z.nbands > katoann_hip::kJoinBand &&
+                   !z.bf16 && !z.full
+               ? 1u
+               : 0u;
   z.nstate = 3 + z.nbands;
   z.max_rows = const_cast<katoann_index &>(ix).max_rows_per_query(z.np);
   // items <= sum over lists of row tiles x (ceil(B / qc) + bands): a list's query
@@ -754,7 +762,7 @@ SearchPlan plan_search(const katoann_index &ix, uint32_t nq, const katoann_searc
                                                      const_cast<katoann_index &>(ix).max_rows_per_query(
                                                          std::min(z.np, 1u + (1u << z.shift))))}));
   if (ix.pool_allowance) z.allowance = std::min<uint64_t>(z.max_rows, ix.pool_allowance);
-  z.akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8), z.rt, z.qc,
+  z.akey = {z.B, z.np, z.S, z.P, z.shift | (uint64_t(z.seed) << 8) | (uint64_t(z.join) << 40), z.rt, z.qc,
             uint64_t(z.variant) | (uint64_t(z.wave) << 8) | (uint64_t(z.ring) << 9) | (uint64_t(z.full) << 10) |
                 (uint64_t(z.bf16) << 11) | (uint64_t(z.qs) << 12) | (uint64_t(z.hyb) << 13) |
                 (uint64_t(z.gemm) << 14)};
@@ -902,9 +910,11 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     }
     katoann_hip::launch_plan(w.dist, w.probes, w.probe_d, npq, caller_npq, B, np, ix.list_off_d.as<uint64_t>(), nlist,
                              qc, rt, z.qsq, z.qsrt, w.pscratch, w.gcnt, w.qoff, w.item_off, w.qlist, w.totals,
-                             w.pool_cnt, w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, s);
+                             w.pool_cnt, w.pool_sorted, w.thr, w.lbound, w.state, nstate, shift, z.seed, z.join, s);
     katoann_hip::launch_items(w.item_off, nslots, nlist, ix.list_off_d.as<uint64_t>(), w.gcnt, w.qoff, qc, rt, z.qsq,
-                              z.qsrt, item_cap, z.seed, w.state + nstate, w.items, s);
+                              z.qsrt, item_cap, z.seed, z.join,
+                              w.pscratch + katoann_hip::plan_join_gb_offset(B, nlist, np, shift, z.seed),
+                              w.state + nstate, w.items, s);
     mark(4);
     const int8_t *plane = prefix ? ix.pre.as<int8_t>() : ix.hi.as<int8_t>();
     const int32_t *xnorm = prefix ? ix.headnorm.as<int32_t>() : ix.hinorm.as<int32_t>();
@@ -913,6 +923,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     const uint32_t qstride = prefix ? P : dc;
     for (uint32_t band = 0; band < nbands; ++band) {
       if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band], s));
+      if (z.join && band == 1) {  // empty: its rows are ranked in band 2, on the seed band's thresholds
+        if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band + 1], s));
+        continue;
+      }
       if (z.bf16)
         katoann_hip::launch_rank_bf16(band, ix.bf.as<uint16_t>(), ix.bfn.as<float>(), ix.ids.as<uint32_t>(), nlist,
                                       w.qbf, w.qbn, ix.dp, w.thr, w.qlist, w.item_off, w.items, stride, w.pool,
@@ -941,8 +955,10 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       if (!launched) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
       if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band + 1], s));
       // after the last band, k_refine (S <= 1,024) takes the S smallest keys
-      // of the whole pool itself: no update of its own
-      if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024) break;
+      // of the whole pool itself: no update of its own (unless the seed
+      // remainder was joined: the last band then keeps many more keys, which
+      // the block-wide update selects faster than the refinement's wave)
+      if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024 && !z.join) break;
       if (!katoann_hip::launch_update(w.pool, w.pool_cnt, w.pool_sorted, w.thr, w.lbound, B, S, stride, P, ix.step_sq,
                                       z.bf16, s))
         throw std::invalid_argument("unsupported survivor count");
@@ -1273,19 +1289,20 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
       if (nb == last_nb || nb > kMaxBands) continue;
       last_nb = nb;
       trial(TileConfig{1u, 0u, 1024u, shift, seed});
+      if (seed && nb > katoann_hip::kJoinBand) trial(TileConfig{1u, 0u, 1024u, shift, seed, 1u});
     }
   }
-  const uint32_t shift = pick.shift, seed = pick.seed;
+  const uint32_t shift = pick.shift, seed = pick.seed, join = pick.join;
   for (uint32_t variant : {0u, 1u, 2u, 3u})
-    for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift, seed});
-  for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift, seed});
+    for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{1u, variant, qc, shift, seed, join});
+  for (uint32_t variant : {0u, 1u}) trial(TileConfig{2u, variant, 512u, shift, seed, join});
   // the query-stationary kernel: rows per item
   for (uint32_t variant : {0u, 1u})
-    for (uint32_t rows : {256u, 1024u, 4096u}) trial(TileConfig{3u, variant, rows, shift, seed});
+    for (uint32_t rows : {256u, 1024u, 4096u}) trial(TileConfig{3u, variant, rows, shift, seed, join});
   // both forms per list (query-stationary for lists with few queries)
-  for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{4u, 0u, qc, shift, seed});
+  for (uint32_t qc : {256u, 512u, 1024u, 4096u}) trial(TileConfig{4u, 0u, qc, shift, seed, join});
   // the block-tiled kernel: rows per item
-  for (uint32_t rows : {512u, 2048u, 8192u}) trial(TileConfig{5u, 0u, rows, shift, seed});
+  for (uint32_t rows : {512u, 2048u, 8192u}) trial(TileConfig{5u, 0u, rows, shift, seed, join});
   std::lock_guard<std::mutex> g(mix.tune_mu);
   mix.tuned[key] = pick;
 }
@@ -1339,6 +1356,7 @@ int katoann_tile_config(const katoann_index *h, uint32_t nq, const katoann_searc
   out[2] = c.qc;
   out[3] = c.shift;
   out[4] = c.seed;
+  out[5] = c.join;
   KATOANN_CATCH
 }
 
@@ -1502,6 +1520,13 @@ int katoann_set_seed_rows(katoann_index *h, uint32_t rows) {
   if (rows > (1u << 20)) throw std::invalid_argument("seed rows must be at most 2^20");
   std::lock_guard<std::mutex> g(h->mu);
   h->seed_rows = rows;
+  KATOANN_CATCH
+}
+
+int katoann_set_seed_join(katoann_index *h, int32_t on) {
+  KATOANN_TRY
+  std::lock_guard<std::mutex> g(h->mu);
+  h->seed_join = on ? 1u : 0u;
   KATOANN_CATCH
 }
 

@@ -1090,7 +1090,8 @@ __global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ 
                                                    uint32_t B, uint32_t np, uint32_t nlist, uint32_t shift,
                                                    const uint32_t *__restrict__ seed_r0, uint32_t seed,
                                                    const uint64_t *__restrict__ list_off, uint32_t *__restrict__ part,
-                                                   const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist) {
+                                                   const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist,
+                                                   uint32_t join, uint32_t *__restrict__ acnt) {
   const uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (e >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(e / np), r = static_cast<uint32_t>(e % np);
@@ -1102,9 +1103,15 @@ __global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ 
   if (FILL) qlist[qoff[slot] + atomicAdd(&part[slot], 1u)] = q;
   else atomicAdd(&part[slot], 1u);
   if (seed && b == 0 && list_off[c + 1] - list_off[c] > seed) {
-    const size_t rest = static_cast<size_t>(nlist) + c;
-    if (FILL) qlist[qoff[rest] + atomicAdd(&part[rest], 1u)] = q;
-    else atomicAdd(&part[rest], 1u);
+    if (join) {  // after the slot's own queries of the first probe-rank band (acnt: count, then cursor)
+      const size_t rest = static_cast<size_t>(katoann_hip::kJoinBand) * nlist + c;
+      if (FILL) qlist[qoff[rest] + atomicAdd(&acnt[c], 1u)] = q;
+      else atomicAdd(&acnt[c], 1u);
+    } else {
+      const size_t rest = static_cast<size_t>(nlist) + c;
+      if (FILL) qlist[qoff[rest] + atomicAdd(&part[rest], 1u)] = q;
+      else atomicAdd(&part[rest], 1u);
+    }
   }
 }
 
@@ -1141,7 +1148,7 @@ __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t q
                                                       uint32_t nlist, const uint64_t *__restrict__ list_off,
                                                       uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
                                                       uint32_t seed, uint32_t *gcnt, uint32_t *nit, uint32_t *bsum,
-                                                      uint32_t *ihist) {
+                                                      uint32_t *ihist, uint32_t join, uint32_t *acnt, uint32_t *gbv) {
   __shared__ uint64_t tmp[17];
   __shared__ uint32_t hist[katoann_hip::kPlanBands * katoann_hip::kItemClasses];
   for (uint32_t i = threadIdx.x; i < katoann_hip::kPlanBands * katoann_hip::kItemClasses; i += 1024) hist[i] = 0;
@@ -1159,21 +1166,35 @@ __global__ __launch_bounds__(1024) void k_slot_totals(uint32_t *part, uint32_t q
     uint64_t first, end;
     katoann_hip::slot_rows(sl / nlist, list_off[c + 1] - list_off[c], seed, first, end);
     const uint32_t n = static_cast<uint32_t>(end - first);
-    it = !(g && n)                    ? 0u
-         : katoann_hip::slot_qs(g, qsq) ? (n + qsrt - 1) / qsrt
-                                        : ((g + qc - 1) / qc) * ((n + rt - 1) / rt);
+    uint32_t *hb = hist + (sl / nlist) * katoann_hip::kItemClasses;
+    // the items per cost class (k_items places them by class) of g queries
+    // in chunks of qc over `rows` rows
+    auto classes = [&](uint32_t gq, uint32_t rows) {
+      if (!gq || !rows) return;
+      const uint32_t nch = (gq + qc - 1) / qc, nt = (rows + rt - 1) / rt;
+      if (nch > 1) atomicAdd(&hb[katoann_hip::item_class(qc)], (nch - 1) * nt);
+      atomicAdd(&hb[katoann_hip::item_class(gq - (nch - 1) * qc)], nt);
+    };
+    const uint32_t ga = join && sl / nlist == katoann_hip::kJoinBand ? acnt[c] : 0u;
+    if (join && sl / nlist == katoann_hip::kJoinBand) {
+      gbv[c] = g;
+      acnt[c] = g;  // the joined queries' cursor starts after the slot's own
+    }
+    if (ga) {  // joined seed remainder: rows [0, n1) x g queries, [n1, n) x g + ga
+      const uint32_t n1 = min(seed, n);
+      it = katoann_hip::chunk_items(g, n1, qc, rt) + katoann_hip::chunk_items(g + ga, n - n1, qc, rt);
+      classes(g, n1);
+      classes(g + ga, n - n1);
+      g += ga;
+    } else if (g && n && katoann_hip::slot_qs(g, qsq)) {
+      it = (n + qsrt - 1) / qsrt;
+      atomicAdd(&hb[katoann_hip::item_class(g)], it);
+    } else {
+      it = katoann_hip::chunk_items(g, n, qc, rt);
+      classes(g, n);
+    }
     gcnt[sl] = g;
     nit[sl] = it;
-    // the slot's items per cost class (k_items places them by class): all
-    // with g queries (query-stationary), or chunks of qc and a last one
-    uint32_t *hb = hist + (sl / nlist) * katoann_hip::kItemClasses;
-    if (it && katoann_hip::slot_qs(g, qsq)) {
-      atomicAdd(&hb[katoann_hip::item_class(g)], it);
-    } else if (it) {
-      const uint32_t nch = (g + qc - 1) / qc, nt = (n + rt - 1) / rt;
-      if (nch > 1) atomicAdd(&hb[katoann_hip::item_class(qc)], (nch - 1) * nt);
-      atomicAdd(&hb[katoann_hip::item_class(g - (nch - 1) * qc)], nt);
-    }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < katoann_hip::kPlanBands * katoann_hip::kItemClasses; i += 1024)
@@ -1399,7 +1420,9 @@ uint32_t plan_slots(uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) 
 size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) {
   const size_t nslots = plan_slots(nlist, np, shift, seed);
   const size_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
-  return (inv_from_probes(nlist, np) ? 1 : qtiles) * nslots + nslots + 2 * nblk + B;
+  // tile partial counts, item counts, block sums, seed ranks, and for a
+  // joined seed remainder its counts / cursors and the slots' own counts
+  return (inv_from_probes(nlist, np) ? 1 : qtiles) * nslots + nslots + 2 * nblk + B + 2ull * nlist;
 }
 
 template <bool FILL>
@@ -1429,12 +1452,13 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                  uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
                  uint32_t qsq, uint32_t qsrt, uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
                  uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr,
-                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s) {
+                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, uint32_t join,
+                 hipStream_t s) {
   const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
   const bool from_probes = inv_from_probes(nlist, np);
   const uint32_t qtiles = from_probes ? 1u : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
-  uint32_t *seed_r0 = bsum + 2 * nblk;
+  uint32_t *seed_r0 = bsum + 2 * nblk, *acnt = seed_r0 + B, *gbv = acnt + nlist;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
   // the state and, after it, the per (band, class) item counts and cursors
   const uint32_t nclear = nstate + katoann_hip::kItemHistWords;
@@ -1448,21 +1472,22 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
     const uint64_t pairs = static_cast<uint64_t>(B) * np;
     const dim3 pgrid(static_cast<uint32_t>((pairs + 255) / 256));
     hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, part, nslots, 0u);
+    if (join) hipLaunchKernelGGL(k_fill32, dim3((nlist + 255) / 256), dim3(256), 0, s, acnt, nlist, 0u);
     hipLaunchKernelGGL(k_inv_probe<false>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
-                       list_off, part, nullptr, nullptr);
+                       list_off, part, nullptr, nullptr, join, acnt);
     // (one partial count per slot: the counts become the slots' cursors, at 0)
     hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
-                       gcnt, nit, bsum, state + nstate);
+                       gcnt, nit, bsum, state + nstate, join, acnt, gbv);
     hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
     hipLaunchKernelGGL(k_inv_probe<true>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
-                       list_off, part, qoff, qlist);
+                       list_off, part, qoff, qlist, join, acnt);
     hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr, lbound);
     return;
   }
   inv_pass_launch<false>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                          nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
-                     gcnt, nit, bsum, state + nstate);
+                     gcnt, nit, bsum, state + nstate, 0u, nullptr, nullptr);
   hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
   inv_pass_launch<true>(nb, tiles, s, dist, probes, probe_d, npq, B, np, nlist, shift, seed_r0, seed, list_off, part,
                         qoff, qlist);

@@ -17,6 +17,20 @@ constexpr uint32_t kCntStride = 32;
 // of qc queries and rt rows as usual.
 constexpr uint32_t kQsItem = 0x80000000u;
 __host__ __device__ inline bool slot_qs(uint32_t g, uint32_t qsq) { return qsq && g <= qsq; }
+// Items of rows x g queries in chunks of qc queries and tiles of rt rows.
+__host__ __device__ inline uint32_t chunk_items(uint32_t g, uint64_t rows, uint32_t qc, uint32_t rt) {
+  return (g && rows) ? ((g + qc - 1) / qc) * static_cast<uint32_t>((rows + rt - 1) / rt) : 0u;
+}
+// Joined seed remainder (join = 1, with a seed band and the probe-list
+// inversion): the queries whose seed band ranked the first R0 rows of a list
+// longer than R0 rank the rest of it in the first probe-rank band (band 2)
+// instead of a band of its own, so those rows are read once with the band's
+// other queries of the list, and band 1 stays empty (no launch, no pool
+// update after it). Such a slot of band 2 lists its gb own queries first,
+// then its ga joined ones: part 1 is rows [0, min(R0, n)) x the gb queries,
+// part 2 rows [min(R0, n), n) x all gb + ga.
+constexpr uint32_t kJoinBand = 2;
+
 // Cost classes of the ranking items. A ranking kernel deals a band's items to
 // its waves in item order, and an item's time grows with the query fragments
 // it streams (16 queries each): dealt in list order, some waves drew far more
@@ -90,7 +104,13 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                  uint32_t B, uint32_t np, const uint64_t *list_off, uint32_t nlist, uint32_t qc, uint32_t rt,
                  uint32_t qsq, uint32_t qsrt, uint32_t *scratch, uint32_t *gcnt, uint32_t *qoff, uint32_t *item_off,
                  uint32_t *qlist, uint32_t *totals, uint32_t *pool_cnt, uint32_t *pool_sorted, uint64_t *thr,
-                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, hipStream_t s);
+                 int32_t *lbound, uint32_t *state, uint32_t nstate, uint32_t shift, uint32_t seed, uint32_t join,
+                 hipStream_t s);
+// The joined seed remainder's per-list counts of a slot's own queries (first
+// probe-rank band), in the plan scratch after the inversion's words.
+inline size_t plan_join_gb_offset(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shift, uint32_t seed) {
+  return plan_scratch_words(B, nlist, np, shift, seed) - nlist;
+}
 void launch_fill32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 // Queries per ranking item for ranking width P (0 if P is unsupported).
 uint32_t rank_queries_per_item(uint32_t P);
@@ -99,7 +119,8 @@ uint32_t rank_queries_per_item(uint32_t P);
 // cursors.
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
                   const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                  uint32_t cap, uint32_t seed, uint32_t *ihist, uint4 *items, hipStream_t s);
+                  uint32_t cap, uint32_t seed, uint32_t join, const uint32_t *gbv, uint32_t *ihist, uint4 *items,
+                  hipStream_t s);
 // Rows [first, end) of list c's n rows that slot band `band` ranks.
 __host__ __device__ inline void slot_rows(uint32_t band, uint64_t n, uint32_t seed, uint64_t &first, uint64_t &end) {
   first = 0;

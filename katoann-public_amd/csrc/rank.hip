@@ -95,7 +95,8 @@ __global__ __launch_bounds__(256) void k_items(const uint32_t *__restrict__ item
                                                uint32_t nlist, const uint64_t *__restrict__ list_off,
                                                const uint32_t *__restrict__ gcnt, const uint32_t *__restrict__ qoff,
                                                uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt, uint32_t cap,
-                                               uint32_t seed, uint32_t *ihist, uint4 *items) {
+                                               uint32_t seed, uint32_t join, const uint32_t *__restrict__ gbv,
+                                               uint32_t *ihist, uint4 *items) {
   constexpr uint32_t NC = katoann_hip::kItemClasses, NH = katoann_hip::kPlanBands * NC;
   __shared__ uint32_t base[NH];   // first position of (band, class) in the items, then this block's run
   __shared__ uint32_t cnt[NH];    // this block's items per (band, class)
@@ -130,20 +131,38 @@ __global__ __launch_bounds__(256) void k_items(const uint32_t *__restrict__ item
     uint64_t first, end;
     katoann_hip::slot_rows(slot / nlist, list_off[c + 1] - list_off[c], seed, first, end);
     const uint64_t n = end - first;
-    const uint32_t local = it - item_off[slot];
-    if (katoann_hip::slot_qs(gcnt[slot], qsq)) {  // all the slot's queries, rows qsrt at a time
+    uint32_t local = it - item_off[slot];
+    // a slot of the first probe-rank band holding its list's joined seed
+    // remainder: part 1 (rows [0, n1), its own gb queries), then part 2 (rows
+    // [n1, n), all gcnt); otherwise one part of gcnt queries over all rows
+    uint32_t g = gcnt[slot];
+    uint64_t r0 = first, r1 = end;
+    const bool joined = join && slot / nlist == katoann_hip::kJoinBand && gbv[c] < g;
+    if (joined) {
+      const uint32_t gb = gbv[c], n1 = static_cast<uint32_t>(min(n, static_cast<uint64_t>(seed)));
+      const uint32_t i1 = katoann_hip::chunk_items(gb, n1, qc, rt);
+      if (local < i1) {
+        g = gb;
+        r1 = first + n1;
+      } else {
+        local -= i1;
+        r0 = first + n1;
+      }
+    }
+    const uint64_t nr = r1 - r0;
+    if (!joined && katoann_hip::slot_qs(g, qsq)) {  // all the slot's queries, rows qsrt at a time
       d.x = qoff[slot];
       d.y = gcnt[slot] | katoann_hip::kQsItem;
       d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(local) * qsrt);
       d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(local + 1) * qsrt));
       key = (slot / nlist) * NC + katoann_hip::item_class(gcnt[slot]);
     } else {  // the query chunks of one row tile are consecutive items
-      const uint32_t nch = (gcnt[slot] + qc - 1) / qc;
+      const uint32_t nch = (g + qc - 1) / qc;
       const uint32_t qch = local % nch, rtile = local / nch;
       d.x = qoff[slot] + qch * qc;
-      d.y = min(qc, gcnt[slot] - qch * qc);
-      d.z = static_cast<uint32_t>(list_off[c] + first + static_cast<uint64_t>(rtile) * rt);
-      d.w = static_cast<uint32_t>(list_off[c] + first + min(n, static_cast<uint64_t>(rtile + 1) * rt));
+      d.y = min(qc, g - qch * qc);
+      d.z = static_cast<uint32_t>(list_off[c] + r0 + static_cast<uint64_t>(rtile) * rt);
+      d.w = static_cast<uint32_t>(list_off[c] + r0 + min(nr, static_cast<uint64_t>(rtile + 1) * rt));
       key = (slot / nlist) * NC + katoann_hip::item_class(d.y);
     }
   }
@@ -2033,8 +2052,7 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
     // updated after the last band: it holds the S sorted survivors of the
     // earlier bands, then the last band's keys)
     uint64_t tq = a.thr[q];
-    auto chunk = [&](uint32_t b) {
-      uint64_t k = b + lane < m ? a.pool[base + b + lane] : kNoKey;
+    auto chunk = [&](uint64_t k) {
       if (k > tq) k = kNoKey;
       const uint64_t mk = ballot(k != kNoKey);
       if (!mk) return;
@@ -2053,11 +2071,27 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
     // The chunks holding the earlier survivors first, then the last band's
     // keys in four interleaved passes (every fourth chunk): the filter
     // tightens on a sample spread over the whole band, whatever order the
-    // ranking kernel appended its keys in.
+    // ranking kernel appended its keys in. Chunks are loaded eight at a time
+    // (unconditional loads, clamped into the pool) so that their round trips
+    // overlap.
     const uint32_t nch = (m + 63) / 64, c0 = min(nch, (a.S + 63) / 64);
-    for (uint32_t c = 0; c < c0; ++c) chunk(64 * c);
-    for (uint32_t p = 0; p < 4; ++p)
-      for (uint32_t c = c0 + p; c < nch; c += 4) chunk(64 * c);
+    auto group = [&](uint32_t cfirst, uint32_t step, uint32_t cend) {
+      for (uint32_t cb = cfirst; cb < cend; cb += 8 * step) {
+        uint64_t kk[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+          const uint32_t i = 64 * (cb + u * step) + lane;
+          kk[u] = a.pool[base + min(i, m - 1)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+          const uint32_t c = cb + u * step;
+          if (c < cend) chunk(64 * c + lane < m ? kk[u] : kNoKey);
+        }
+      }
+    };
+    group(0, 1, c0);
+    for (uint32_t p = 0; p < 4; ++p) group(c0 + p, 4, nch);
   }
   const uint32_t ns = top_count<SC>(sv);
   if (a.surv) {
@@ -2421,9 +2455,10 @@ uint32_t rank_queries_per_item(uint32_t P) {
 
 void launch_items(const uint32_t *item_off, uint32_t nslots, uint32_t nlist, const uint64_t *list_off,
                   const uint32_t *gcnt, const uint32_t *qoff, uint32_t qc, uint32_t rt, uint32_t qsq, uint32_t qsrt,
-                  uint32_t cap, uint32_t seed, uint32_t *ihist, uint4 *items, hipStream_t s) {
+                  uint32_t cap, uint32_t seed, uint32_t join, const uint32_t *gbv, uint32_t *ihist, uint4 *items,
+                  hipStream_t s) {
   hipLaunchKernelGGL(k_items, dim3((cap + 255) / 256), dim3(256), 0, s, item_off, nslots, nlist, list_off, gcnt, qoff,
-                     qc, rt, qsq, qsrt, cap, seed, ihist, items);
+                     qc, rt, qsq, qsrt, cap, seed, join, gbv, ihist, items);
 }
 
 template <int KC>

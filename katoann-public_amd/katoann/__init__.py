@@ -118,6 +118,7 @@ def lib():
         L.katoann_set_rank_rows.argtypes = [P, u32]
         L.katoann_set_band_factor.argtypes = [P, u32]
         L.katoann_set_seed_rows.argtypes = [P, u32]
+        L.katoann_set_seed_join.argtypes = [P, i32]
         L.katoann_set_pool_allowance.argtypes = [P, u32]
         L.katoann_set_rank_kernel.argtypes = [P, u32, u32, u32]
         L.katoann_set_graphs.argtypes = [P, i32]
@@ -480,6 +481,12 @@ class Index:
         every query has a threshold early; 0 = no seed band."""
         _check(lib().katoann_set_seed_rows(self._h, rows))
 
+    def set_seed_join(self, on: bool = True):
+        """With a seed band and nprobe <= nlist / 16: rank the rest of each
+        query's seed lists in the first probe-rank band, with that band's
+        queries of the same lists (rows read once, one pool update less)."""
+        _check(lib().katoann_set_seed_join(self._h, int(on)))
+
     def set_pool_allowance(self, slots: int):
         """Survivor pool room per query beyond S (0 = sized per search); a band
         that keeps more reruns the batch with 4x the room."""
@@ -502,12 +509,12 @@ class Index:
 
     def tile_config(self, nq, k=10, nprobe=64, ranking="prefix", survivors=32, selection="threshold"):
         """The tile (kernel, variant, queries per item, log2 band factor, seed
-        rows) a search of this shape runs with."""
-        out = np.zeros(5, np.uint32)
+        rows, joined seed remainder) a search of this shape runs with."""
+        out = np.zeros(6, np.uint32)
         prm = self._params(k, nprobe, ranking, survivors, selection)
         _check(lib().katoann_tile_config(self._h, nq, C.byref(prm), _ptr(out)))
         return dict(kernel=int(out[0]), variant=int(out[1]), queries_per_item=int(out[2]), log2_band_factor=int(out[3]),
-                    seed_rows=int(out[4]))
+                    seed_rows=int(out[4]), seed_join=int(out[5]))
 
     def set_graphs(self, on: bool = True):
         """Replay each search's kernels from a captured hipGraph (default on)."""

@@ -126,6 +126,32 @@ def test_seed_band_vs_oracle(mixture_index, wide_index, seed, shift):
         check_against_oracle(ix, ox, q[:300], 10, min(32, ix.nlist), 32, npq=npq)
 
 
+@pytest.mark.parametrize("seed,shift", [(32, 2), (64, 1), (500, 3), (5000, 12)])
+def test_seed_join_vs_oracle(mixture_index, seed, shift):
+    """The seed remainder joined to the first probe-rank band (its rows ranked
+    once with that band's queries, on the seed band's thresholds): survivors
+    and results are the oracle's, with per-query nprobe and an overflowing
+    pool too. nprobe <= nlist / 16 here, so the join is taken."""
+    rng = np.random.default_rng(seed + 1)
+    prefix, base, q = mixture_index
+    ix = load(prefix)
+    ix.set_seed_rows(seed)
+    ix.set_band_factor(shift)
+    ix.set_seed_join(True)
+    ox = _oracle(prefix)
+    for (k, nprobe, S) in ((10, 16, 32), (10, 4, 128), (100, 8, 100), (10, 16, 300)):
+        assert ix.tile_config(300, k=k, nprobe=nprobe, survivors=S)["seed_join"] == 1
+        print("seed join", seed, shift, k, nprobe, S, flush=True)
+        check_against_oracle(ix, ox, q[:300], k, nprobe, S)
+    npq = rng.integers(1, 17, 300).astype(np.uint32)
+    print("seed join per-query nprobe", seed, shift, flush=True)
+    check_against_oracle(ix, ox, q[:300], 10, 16, 32, npq=npq)
+    ix.set_pool_allowance(8)
+    for rep in range(2):
+        print("seed join overflow rerun", seed, shift, rep, flush=True)
+        check_against_oracle(ix, ox, q[:300], 10, 16, 32)
+
+
 def test_pool_overflow_reruns(mixture_index):
     """A pool too small for a band's kept keys: the batch runs again with more
     room (and later searches of the shape start from it); results stay the
