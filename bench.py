@@ -308,7 +308,7 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
         ix.set_rank_kernel(1, 0, 1024)
         ix.set_band_factor(5)
         ix.set_seed_rows(1024)
-        ix.set_seed_join(0)
+        ix.set_seed_join(1)
         ix.set_autotune(True)
     rank_ms = st["rank"]
     byts, ops = head["rank_bytes"], head["rank_ops"]

@@ -258,7 +258,7 @@ int katoann_set_seed_rows(katoann_index *h, uint32_t rows);
  * and nprobe <= nlist / 16, the rest of each query's seed lists (rows past R0)
  * is ranked in the first probe-rank band with that band's queries of the same
  * list, on the thresholds the seed band produced, instead of in a band of its
- * own: those rows are read once and one pool update goes. Default off.
+ * own: those rows are read once and one pool update goes. Default on.
  * Results do not depend on it. */
 int katoann_set_seed_join(katoann_index *h, int32_t on);
 /* Survivor pool room per query beyond S for one band's kept keys (no

@@ -129,7 +129,7 @@ struct katoann_index {
   uint32_t rank_rows = 1024;
   uint32_t band_shift = 5;     // probe-rank bands grow by 2^band_shift
   uint32_t seed_rows = 1024;  // rows of the seed band (0: none)
-  uint32_t seed_join = 0;     // seed remainder joined to the first probe-rank band
+  uint32_t seed_join = 1;     // seed remainder joined to the first probe-rank band (where it applies)
   uint32_t pool_allowance = 0;  // pool slots per query beyond S (0: sized per search)
   uint32_t rank_kernel = 1, wave_variant = 0, wave_qc = 1024;
   std::atomic<bool> graphs{true};  // replay each search's kernels from a captured graph
@@ -955,10 +955,8 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       if (!launched) throw std::invalid_argument("unsupported ranking shape (prefix dims)");
       if (timing) HIPCHECK(hipEventRecord(ctx->bev[2 * band + 1], s));
       // after the last band, k_refine (S <= 1,024) takes the S smallest keys
-      // of the whole pool itself: no update of its own (unless the seed
-      // remainder was joined: the last band then keeps many more keys, which
-      // the block-wide update selects faster than the refinement's wave)
-      if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024 && !z.join) break;
+      // of the whole pool itself: no update of its own
+      if (band + 1 == nbands && !z.bf16 && katoann_hip::refine_cap(S) <= 1024) break;
       if (!katoann_hip::launch_update(w.pool, w.pool_cnt, w.pool_sorted, w.thr, w.lbound, B, S, stride, P, ix.step_sq,
                                       z.bf16, s))
         throw std::invalid_argument("unsupported survivor count");
@@ -1123,7 +1121,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
       float rank = 0.0f;
       for (uint32_t b = 0; b < kMaxBands; ++b) {
         float t = 0.0f;
-        if (b < nbands) (void)hipEventElapsedTime(&t, ctx->bev[2 * b], ctx->bev[2 * b + 1]);
+        if (b < nbands && !(z.join && b == 1)) (void)hipEventElapsedTime(&t, ctx->bev[2 * b], ctx->bev[2 * b + 1]);
         mix.band_ms[b] = t;
         rank += t;
       }
@@ -1133,7 +1131,7 @@ void search_impl(const katoann_index &ix, const float *queries, uint32_t nq, con
     mix.work[0] = st[4];  // (query, list) pairs
     mix.work[1] = st[5];  // ranking items
     mix.work[2] = st[1];  // candidates kept
-    mix.work[3] = nbands;
+    mix.work[3] = nbands - z.join;  // ranking launches (a joined remainder's band has none)
   }
 }
 
@@ -1269,7 +1267,7 @@ void maybe_tune(const katoann_index &ix, const float *queries, uint32_t nq, cons
       return;
     search_impl(ix, queries, nq, p, x, ids, dists, &c);
     double dt = 1e30;
-    for (int rep = 0; rep < 2; ++rep) {  // the better of two timed runs
+    for (int rep = 0; rep < 4; ++rep) {  // the best of four timed runs (single runs vary by a few per cent)
       const auto t0 = std::chrono::steady_clock::now();
       search_impl(ix, queries, nq, p, x, ids, dists, &c);
       dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());

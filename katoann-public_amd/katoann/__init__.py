@@ -458,10 +458,11 @@ class Index:
         return dict(zip(STAGES, out.tolist()))
 
     def band_times(self):
-        """Ranking-launch ms of each probe-rank band of the last timed search."""
+        """Ranking-launch ms of each launched probe-rank band of the last
+        timed search (a joined seed remainder's band is not launched)."""
         out = np.zeros(16, np.float32)
         _check(lib().katoann_band_times(self._h, _ptr(out), 16))
-        return out[:self.last_work()["bands"]].tolist()
+        return [float(t) for t in out if t > 0]
 
     def last_work(self):
         out = np.zeros(4, np.uint64)

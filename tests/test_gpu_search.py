@@ -117,6 +117,7 @@ def test_seed_band_vs_oracle(mixture_index, wide_index, seed, shift):
         ix = load(prefix)
         ix.set_seed_rows(seed)
         ix.set_band_factor(shift)
+        ix.set_seed_join(False)  # the remainder in a band of its own (test_seed_join_vs_oracle: joined)
         ox = _oracle(prefix)
         for (k, nprobe, S) in ((10, 16, 32), (10, 4, 128), (100, 8, 100)):
             print("seed", seed, shift, name, k, nprobe, S, flush=True)
