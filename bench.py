@@ -303,6 +303,7 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
                          survivors=head["survivors"])
         st += np.array(list(ix.stage_times().values()))
     ix.set_timing(False)
+    launches = ix.last_work()["bands"]  # ranking launches of the timed tile
     st = dict(zip(katoann.STAGES, (st / args.steps).tolist()))
     if prof:  # back to the index's defaults with tuning for what follows
         ix.set_rank_kernel(1, 0, 1024)
@@ -326,7 +327,7 @@ def roofline_pass(ix, queries, head, sizes, pd, d, nlist, name, args):
            "traffic_profile": prof["profile"] if prof else None, "tile": tile,
            "intensity_ops_per_byte": inten, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
            "int8_tops": tops, "int8_frac": tops / MFMA_I8_PEAK_TOPS, "bytes_per_search": byts,
-           "ops_per_search": ops, "ms_per_search": rank_ms, "launches_per_search": head["work"]["bands"],
+           "ops_per_search": ops, "ms_per_search": rank_ms, "launches_per_search": launches,
            "bytes_rule": "sum over touched lists of rows x (prefix bytes + 4-byte norm), once per search",
            "ops_rule": "2 x prefix bytes per (query, row of a probed list)",
            "timing": "HIP events on the search stream around every ranking launch, averaged over the steps",

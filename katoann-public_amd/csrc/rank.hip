@@ -2052,21 +2052,31 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
     // updated after the last band: it holds the S sorted survivors of the
     // earlier bands, then the last band's keys)
     uint64_t tq = a.thr[q];
+    // Keys below the filter wait in the wave's LDS area (ex[w], free until the
+    // survivors are known) until 64 have gathered, and are merged into the list
+    // 64 at a time: a merge is a chain of dependent lane exchanges, and most
+    // chunks pass only a few keys once the filter has tightened.
+    uint64_t *pend = ex[w];
+    uint32_t pc = 0;  // keys waiting (wave-uniform)
+    const uint64_t below = (1ull << lane) - 1ull;
     auto chunk = [&](uint64_t k) {
       if (k > tq) k = kNoKey;
       const uint64_t mk = ballot(k != kNoKey);
       if (!mk) return;
-      if (__popcll(mk) <= 4) {
-        uint64_t mm = mk;
-        while (mm) {
-          const int bb = __ffsll(static_cast<long long>(mm)) - 1;
-          mm &= mm - 1;
-          top_insert<SC>(sv, __shfl(k, bb), a.S);
-        }
-      } else {
-        top_merge<SC>(sv, k, a.S);
+      const uint32_t nk = static_cast<uint32_t>(__popcll(mk));
+      const uint32_t at = pc + static_cast<uint32_t>(__popcll(mk & below));
+      if (k != kNoKey && at < 64) pend[at] = k;
+      if (pc + nk < 64) {
+        pc += nk;
+        return;
       }
+      wave_fence();
+      top_merge<SC>(sv, pend[lane], a.S);
       tq = umin64(tq, top_get<SC>(sv, a.S - 1));
+      wave_fence();
+      if (k != kNoKey && at >= 64) pend[at - 64] = k;
+      pc = pc + nk - 64;
+      wave_fence();
     };
     // The chunks holding the earlier survivors first, then the last band's
     // keys in four interleaved passes (every fourth chunk): the filter
@@ -2092,6 +2102,11 @@ __global__ __launch_bounds__(256, 2) void k_refine(RefineArgs a) {
     };
     group(0, 1, c0);
     for (uint32_t p = 0; p < 4; ++p) group(c0 + p, 4, nch);
+    if (pc) {
+      wave_fence();
+      top_merge<SC>(sv, static_cast<uint32_t>(lane) < pc ? pend[lane] : kNoKey, a.S);
+    }
+    wave_fence();
   }
   const uint32_t ns = top_count<SC>(sv);
   if (a.surv) {
