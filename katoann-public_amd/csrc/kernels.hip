@@ -743,27 +743,6 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
 #pragma unroll
     for (int i = 0; i < V; ++i) v[i] = c0 + i < nlist ? row[c0 + i] : 0xffffffffu;
   }
-  // range of the row (lists past nlist hold 0xffffffff, above every distance)
-  uint32_t vmin = 0xffffffffu, vmax = 0;
-#pragma unroll
-  for (int i = 0; i < V; ++i)
-    if (c0 + i < nlist) {
-      vmin = min(vmin, v[i]);
-      vmax = max(vmax, v[i]);
-    }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), o)));
-    vmax = max(vmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmax), o)));
-  }
-  if (lane == 0) {
-    tmp[w] = vmin;
-    tmp[4 + w] = vmax;
-  }
-  __syncthreads();
-  vmin = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
-  vmax = max(max(tmp[4], tmp[5]), max(tmp[6], tmp[7]));
-  __syncthreads();
   // the kept keys, sorted: runs of W keys by the waves, then merge levels;
   // the first np go out
   auto sort_and_emit = [&](uint32_t M) {
@@ -775,17 +754,23 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
     }
   };
   if (np <= 256) {
-    // Each wave sorts its threads' minimum keys (one per lane) and takes the
-    // ceil(np / 4)-th; at least np keys lie at or below the largest of the
-    // four, and for rows without long runs of small values in few threads
-    // not many more: those candidates hold the answer. Past SEL_FAST
-    // candidates the radix path below runs instead.
-    uint64_t mk = kNoKey;
+    // Each wave finds the ceil(np / 4)-th smallest of its threads' minimum
+    // distances (one per lane; bit by bit from the top, one ballot per bit);
+    // at least np keys lie at or below the largest of the four, and for rows
+    // without long runs of small values in few threads not many more: those
+    // candidates hold the answer. Past SEL_FAST candidates the radix path
+    // below runs instead.
+    uint32_t mv = 0xffffffffu;
 #pragma unroll
     for (int i = 0; i < V; ++i)
-      if (c0 + i < nlist) mk = umin64(mk, (static_cast<uint64_t>(v[i]) << 32) | (c0 + i));
-    mk = wave_sort64(mk);
-    const uint64_t wk = static_cast<uint64_t>(__shfl(static_cast<long long>(mk), static_cast<int>((np + 3) / 4 - 1)));
+      if (c0 + i < nlist) mv = min(mv, v[i]);
+    const uint32_t K = (np + 3) / 4;
+    uint32_t T = 0;  // bits above b of the K-th smallest
+    for (int b = 31; b >= 0; --b) {
+      const uint32_t hi0 = T | ((1u << b) - 1u);  // largest value with bit b clear under the prefix
+      if (static_cast<uint32_t>(__popcll(ballot(mv <= hi0))) < K) T |= 1u << b;
+    }
+    const uint64_t wk = (static_cast<uint64_t>(T) << 32) | 0xffffffffull;
     if (lane == 0) keys[w] = wk;
     __syncthreads();
     const uint64_t tk = umax64(umax64(keys[0], keys[1]), umax64(keys[2], keys[3]));
@@ -813,6 +798,27 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
     }
     __syncthreads();
   }
+  // range of the row (lists past nlist hold 0xffffffff, above every distance)
+  uint32_t vmin = 0xffffffffu, vmax = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    if (c0 + i < nlist) {
+      vmin = min(vmin, v[i]);
+      vmax = max(vmax, v[i]);
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), o)));
+    vmax = max(vmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmax), o)));
+  }
+  if (lane == 0) {
+    tmp[w] = vmin;
+    tmp[4 + w] = vmax;
+  }
+  __syncthreads();
+  vmin = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
+  vmax = max(max(tmp[4], tmp[5]), max(tmp[6], tmp[7]));
+  __syncthreads();
   // radix select of the np-th smallest on r = v - vmin, 11-bit digits
   const uint32_t range = vmax - vmin;
   const int nbits = range ? 32 - __clz(range) : 0;
@@ -915,6 +921,11 @@ __global__ void k_check_npq(uint32_t *npq, uint32_t B, uint32_t np, uint32_t *st
 }
 
 constexpr int INV_QT = 64;    // queries per tile
+// The probe-list inversion counts a slot's pairs on INV_PG counters (the
+// pair's block picks one), so that the thousands of pairs of a popular list
+// do not all serialise on one address; the counters' offsets within the slot
+// are then scanned like the matrix path's per-tile counts.
+constexpr uint32_t INV_PG = 8;
 constexpr int INV_MAXB = 16;  // bands at most (nprobe 4096, F = 2: 13)
 
 __device__ __forceinline__ uint64_t probe_key(const uint32_t *probes, const float *probe_d, size_t i) {
@@ -1091,8 +1102,9 @@ __global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ 
                                                    const uint32_t *__restrict__ seed_r0, uint32_t seed,
                                                    const uint64_t *__restrict__ list_off, uint32_t *__restrict__ part,
                                                    const uint32_t *__restrict__ qoff, uint32_t *__restrict__ qlist,
-                                                   uint32_t join, uint32_t *__restrict__ acnt) {
+                                                   uint32_t join, uint32_t *__restrict__ acnt, uint32_t nslots) {
   const uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  uint32_t *pc = part + static_cast<size_t>(blockIdx.x % INV_PG) * nslots;  // this block's counters
   if (e >= static_cast<uint64_t>(B) * np) return;
   const uint32_t q = static_cast<uint32_t>(e / np), r = static_cast<uint32_t>(e % np);
   const uint32_t n = npq ? min(npq[q], np) : np;
@@ -1100,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ 
   const uint32_t c = probes[e];
   const uint32_t b = rank_band(r, shift, seed, seed ? seed_r0[q] : 0u);
   const size_t slot = static_cast<size_t>(b) * nlist + c;
-  if (FILL) qlist[qoff[slot] + atomicAdd(&part[slot], 1u)] = q;
-  else atomicAdd(&part[slot], 1u);
+  if (FILL) qlist[qoff[slot] + atomicAdd(&pc[slot], 1u)] = q;
+  else atomicAdd(&pc[slot], 1u);
   if (seed && b == 0 && list_off[c + 1] - list_off[c] > seed) {
     if (join) {  // after the slot's own queries of the first probe-rank band (acnt: count, then cursor)
       const size_t rest = static_cast<size_t>(katoann_hip::kJoinBand) * nlist + c;
@@ -1109,8 +1121,8 @@ __global__ __launch_bounds__(256) void k_inv_probe(const uint32_t *__restrict__ 
       else atomicAdd(&acnt[c], 1u);
     } else {
       const size_t rest = static_cast<size_t>(nlist) + c;
-      if (FILL) qlist[qoff[rest] + atomicAdd(&part[rest], 1u)] = q;
-      else atomicAdd(&part[rest], 1u);
+      if (FILL) qlist[qoff[rest] + atomicAdd(&pc[rest], 1u)] = q;
+      else atomicAdd(&pc[rest], 1u);
     }
   }
 }
@@ -1422,7 +1434,7 @@ size_t plan_scratch_words(uint32_t B, uint32_t nlist, uint32_t np, uint32_t shif
   const size_t qtiles = (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   // tile partial counts, item counts, block sums, seed ranks, and for a
   // joined seed remainder its counts / cursors and the slots' own counts
-  return (inv_from_probes(nlist, np) ? 1 : qtiles) * nslots + nslots + 2 * nblk + B + 2ull * nlist;
+  return (inv_from_probes(nlist, np) ? INV_PG : qtiles) * nslots + nslots + 2 * nblk + B + 2ull * nlist;
 }
 
 template <bool FILL>
@@ -1456,7 +1468,7 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
                  hipStream_t s) {
   const uint32_t nb = plan_bands(np, shift, seed), nslots = nb * nlist;
   const bool from_probes = inv_from_probes(nlist, np);
-  const uint32_t qtiles = from_probes ? 1u : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
+  const uint32_t qtiles = from_probes ? INV_PG : (B + INV_QT - 1) / INV_QT, nblk = (nslots + 1023) / 1024;
   uint32_t *part = scratch, *nit = scratch + static_cast<size_t>(qtiles) * nslots, *bsum = nit + nslots;
   uint32_t *seed_r0 = bsum + 2 * nblk, *acnt = seed_r0 + B, *gbv = acnt + nlist;
   const dim3 tiles((nlist + INV_LB - 1) / INV_LB, qtiles);
@@ -1471,16 +1483,17 @@ void launch_plan(const float *dist, const uint32_t *probes, const float *probe_d
   if (from_probes) {
     const uint64_t pairs = static_cast<uint64_t>(B) * np;
     const dim3 pgrid(static_cast<uint32_t>((pairs + 255) / 256));
-    hipLaunchKernelGGL(k_fill32, dim3((nslots + 255) / 256), dim3(256), 0, s, part, nslots, 0u);
+    hipLaunchKernelGGL(k_fill32, dim3((qtiles * nslots + 255) / 256), dim3(256), 0, s, part, qtiles * nslots, 0u);
     if (join) hipLaunchKernelGGL(k_fill32, dim3((nlist + 255) / 256), dim3(256), 0, s, acnt, nlist, 0u);
     hipLaunchKernelGGL(k_inv_probe<false>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
-                       list_off, part, nullptr, nullptr, join, acnt);
-    // (one partial count per slot: the counts become the slots' cursors, at 0)
-    hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, 1u, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
+                       list_off, part, nullptr, nullptr, join, acnt, nslots);
+    // (INV_PG partial counts per slot: the counts become the counters'
+    // cursors, at their offsets within the slot)
+    hipLaunchKernelGGL(k_slot_totals, dim3(nblk), dim3(1024), 0, s, part, qtiles, nslots, nlist, list_off, qc, rt, qsq, qsrt, seed,
                        gcnt, nit, bsum, state + nstate, join, acnt, gbv);
     hipLaunchKernelGGL(k_slot_scan, dim3(nblk), dim3(1024), 0, s, gcnt, nit, bsum, nslots, qoff, item_off, totals);
     hipLaunchKernelGGL(k_inv_probe<true>, pgrid, dim3(256), 0, s, probes, npq, B, np, nlist, shift, seed_r0, seed,
-                       list_off, part, qoff, qlist, join, acnt);
+                       list_off, part, qoff, qlist, join, acnt, nslots);
     hipLaunchKernelGGL(k_query_init, dim3((B + 255) / 256), dim3(256), 0, s, B, pool_cnt, pool_sorted, thr, lbound);
     return;
   }
