@@ -1709,12 +1709,12 @@ constexpr int UPD_R = 16;           // pool keys per thread on the selection pat
 // the others (small pools merge faster: the batches past the first S keys are
 // mostly dropped by the cap before any sorting).
 __device__ __forceinline__ bool update_by_selection(uint32_t S, uint32_t n) {
-  return S <= 256 && n > max(4 * S, 1024u) && n <= 256u * UPD_R;
+  return S <= 256 && n > max(2 * S, 256u) && n <= 256u * UPD_R;
 }
 
-// Selection path: at least S keys lie at or below the largest of the waves'
-// ceil(S / 4)-th smallest thread minimum (each wave sorts its 64 threads'
-// minima, one per lane), and for pools in no particular order not many more;
+// Selection path (pools over max(2 S, 256) keys): at least S keys lie at or
+// below the largest of the waves' ceil(S / 4)-th smallest thread minimum (one
+// per lane), and for pools in no particular order not many more;
 // those candidates, sorted by the four waves, give the S smallest. A query
 // with more than UPD_CAP candidates is left to k_update (run after it).
 // Each thread's keys are loaded at once (independent loads) into registers;
@@ -1734,11 +1734,16 @@ __global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
     const uint32_t i = tid + 256u * r;
     v[r] = i < n ? pq[i] : kNoKey;
   }
-  uint64_t mk = kNoKey;
+  // the wave's ceil(S / 4)-th smallest thread minimum (distance bits), found
+  // bit by bit from the top with one ballot per bit
+  uint32_t mv = 0xffffffffu;
 #pragma unroll
-  for (int r = 0; r < UPD_R; ++r) mk = umin64(mk, v[r]);
-  mk = wave_sort64(mk);
-  const uint64_t wk = static_cast<uint64_t>(__shfl(static_cast<long long>(mk), static_cast<int>((a.S + 3) / 4 - 1)));
+  for (int r = 0; r < UPD_R; ++r) mv = min(mv, static_cast<uint32_t>(v[r] >> 32));
+  const uint32_t K = (a.S + 3) / 4;
+  uint32_t T = 0;
+  for (int b = 31; b >= 0; --b)
+    if (static_cast<uint32_t>(__popcll(ballot(mv <= (T | ((1u << b) - 1u))))) < K) T |= 1u << b;
+  const uint64_t wk = (static_cast<uint64_t>(T) << 32) | 0xffffffffull;
   if (lane == 0) swk[w] = wk;
   __syncthreads();
   const uint64_t tk = umax64(umax64(swk[0], swk[1]), umax64(swk[2], swk[3]));
