@@ -372,7 +372,8 @@ def learned_leg(ix, base, queries, gt, b, sizes, args, plain, more, ntrain, nlis
     # the model's nprobe ceiling sets the probe selection's width (a top-nmax
     # per query before the model runs): each candidate ceiling is calibrated on
     # the validation queries, the fastest that reaches recall 0.99 is kept
-    ceilings = [args.learned_nmax] if args.learned_nmax else [c for c in (256, 512, W_nmax) if c <= W_nmax]
+    ceilings = ([args.learned_nmax] if args.learned_nmax
+                else [c for c in (plain["nprobe"], 256, 512, W_nmax) if c <= W_nmax])
     used = torch.empty(nq, dtype=torch.int32, device=dev)
     ids_d = torch.empty((nq, args.k), dtype=torch.int32, device=dev)
     dists_d = torch.empty((nq, args.k), dtype=torch.float32, device=dev)
