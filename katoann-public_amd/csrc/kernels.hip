@@ -473,14 +473,34 @@ __global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ q
   const uint32_t nk = dc / 64;
   const uint32_t ntile_b = blockIdx.x < ntiles ? (ntiles - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
   const uint32_t total = ntile_b * nk;  // chunks of this block
-  // copies of chunk g (chunk g % nk of the block's tile g / nk) into stage g % CO_NS
-  auto issue = [&](uint32_t g) {
-    const uint32_t kc = g % nk, t = blockIdx.x + (g / nk) * gridDim.x;
-    const uint32_t q0 = (t / nct) * CO_T, c0 = (t % nct) * CO_T;
+  // Chunk bookkeeping by increments (the tile's row and column blocks advance
+  // by gridDim.x tiles with a carry): no integer division per chunk, whose
+  // scalar sequences would delay every chunk's copies and products.
+  const uint32_t gq = gridDim.x / nct, gr = gridDim.x % nct;
+  struct Cursor {
+    uint32_t kc, tq, tc;  // chunk within the tile, the tile's query and centroid blocks
+  };
+  auto advance = [&](Cursor &u) {
+    if (++u.kc == nk) {
+      u.kc = 0;
+      u.tq += gq;
+      u.tc += gr;
+      if (u.tc >= nct) {
+        u.tc -= nct;
+        ++u.tq;
+      }
+    }
+  };
+  Cursor ci{0u, blockIdx.x / nct, blockIdx.x % nct};  // next chunk to copy
+  Cursor cc = ci;                                     // next chunk to multiply
+  uint32_t issued = 0, firsts = 0;  // chunks copied; tiles' first chunks among those not yet multiplied
+  // copies of chunk `issued` into stage issued % CO_NS
+  auto issue = [&]() {
+    const uint32_t kc = ci.kc, q0 = ci.tq * CO_T, c0 = ci.tc * CO_T;
     const uint32_t r = 16u * w + (lane & 15);
     const uint32_t qr = min(q0 + r, B - 1), cr = min(c0 + r, nlist - 1);
     const uint32_t off = kc * 64 + 16 * (lane >> 4);
-    int8_t *S = co_st + (g % CO_NS) * CO_STAGE + w * 1024;
+    int8_t *S = co_st + (issued % CO_NS) * CO_STAGE + w * 1024;
     co_dma16(qhi + static_cast<size_t>(qr) * dc + off, S);
     co_dma16(qlo + static_cast<size_t>(qr) * dc + off, S + CO_T * 64);
     co_dma16(chi + static_cast<size_t>(cr) * dc + off, S + 2 * CO_T * 64);
@@ -488,21 +508,24 @@ __global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ q
     if (kc == 0) {  // dword j = 64 w + lane of [query norms | centroid norms]
       const uint32_t j = 64u * w + lane, e = (j & 255u) >> 1;
       const int64_t *src = j < 256 ? qnorm + min(q0 + e, B - 1) : cnorm + min(c0 + e, nlist - 1);
-      co_dma4(reinterpret_cast<const uint32_t *>(src) + (j & 1u), co_st + (g % CO_NS) * CO_STAGE + CO_PLANES + 256 * w);
+      co_dma4(reinterpret_cast<const uint32_t *>(src) + (j & 1u),
+              co_st + (issued % CO_NS) * CO_STAGE + CO_PLANES + 256 * w);
+      ++firsts;
     }
+    advance(ci);
+    ++issued;
   };
   const uint32_t pro = min(static_cast<uint32_t>(CO_NS - 1), total);
-  for (uint32_t g = 0; g < pro; ++g) issue(g);
-  uint32_t issued = pro;
+  while (issued < pro) issue();
   v4i hh[4][2], mid[4][2], ll[4][2];
   int64_t qn[4][4], cn[2];
   for (uint32_t g = 0; g < total; ++g) {
-    const uint32_t kc = g % nk;
-    uint32_t younger = 0;  // copies of the chunks issued after g
-    for (uint32_t h = g + 1; h < issued; ++h) younger += (h % nk == 0) ? 5u : 4u;
-    co_wait_vm(younger);
+    const uint32_t kc = cc.kc;
+    // copies of the chunks issued after g: 4 each, 5 for a tile's first
+    if (kc == 0) --firsts;
+    co_wait_vm(4u * (issued - g - 1) + firsts);
     __builtin_amdgcn_s_barrier();
-    if (issued < total) issue(issued++);
+    if (issued < total) issue();
     const uint32_t S = co_lds_addr(co_st + (g % CO_NS) * CO_STAGE);
     if (kc == 0) {
 #pragma unroll
@@ -535,8 +558,7 @@ __global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ q
         ll[a][t] = mfma(al[a], bl[t], ll[a][t]);
       }
     if (kc == nk - 1) {
-      const uint32_t tile = blockIdx.x + (g / nk) * gridDim.x;
-      const uint32_t q0 = (tile / nct) * CO_T, c0 = (tile % nct) * CO_T;
+      const uint32_t q0 = cc.tq * CO_T, c0 = cc.tc * CO_T;
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const uint32_t c = c0 + 32 * wc + 16 * t + (lane & 15);
@@ -552,6 +574,7 @@ __global__ __launch_bounds__(512, 1) void k_coarse8(const int8_t *__restrict__ q
           }
       }
     }
+    advance(cc);
   }
 }
 
