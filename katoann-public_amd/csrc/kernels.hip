@@ -739,7 +739,7 @@ __global__ __launch_bounds__(256) void k_select(const float *__restrict__ dist, 
 // in registers, and two merge-path levels finish the sort.
 constexpr uint32_t SEL_FAST = 1024;  // candidates of the small-np path at most
 
-template <int V>
+template <int V, bool COAL>
 __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
                                                     uint32_t np, uint32_t *__restrict__ probes,
                                                     float *__restrict__ probe_d) {
@@ -753,19 +753,41 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
   const uint32_t *row = reinterpret_cast<const uint32_t *>(dist) + static_cast<size_t>(q) * nlist;
   const uint32_t c0 = static_cast<uint32_t>(tid) * V;
   uint32_t v[V];
-  if ((nlist & 3u) == 0 && c0 + V <= nlist) {
+  // Small np on a full row: the wave reads 1 KB runs (lane-contiguous 16-byte
+  // pieces; thread t's value i is list 4 (256 (i / 4) + t) + i % 4): a few
+  // cache lines per load instead of one per lane. The small-np path only
+  // compares (value, list) keys, so any ownership serves; the radix path
+  // below reloads the row thread-contiguous.
+  // (COAL: np <= 256 and nlist = 256 V, chosen at launch)
+  auto lid = [&](int i) -> uint32_t {
+    return COAL ? 4u * (256u * static_cast<uint32_t>(i >> 2) + static_cast<uint32_t>(tid)) + static_cast<uint32_t>(i & 3)
+                : c0 + static_cast<uint32_t>(i);
+  };
+  auto load_row = [&](bool lanes) {
+    if (lanes) {
 #pragma unroll
-    for (int i = 0; i < V; i += 4) {
-      const uint4 x = *reinterpret_cast<const uint4 *>(row + c0 + i);
-      v[i] = x.x;
-      v[i + 1] = x.y;
-      v[i + 2] = x.z;
-      v[i + 3] = x.w;
+      for (int i = 0; i < V; i += 4) {
+        const uint4 x = *reinterpret_cast<const uint4 *>(row + lid(i));
+        v[i] = x.x;
+        v[i + 1] = x.y;
+        v[i + 2] = x.z;
+        v[i + 3] = x.w;
+      }
+    } else if ((nlist & 3u) == 0 && c0 + V <= nlist) {
+#pragma unroll
+      for (int i = 0; i < V; i += 4) {
+        const uint4 x = *reinterpret_cast<const uint4 *>(row + c0 + i);
+        v[i] = x.x;
+        v[i + 1] = x.y;
+        v[i + 2] = x.z;
+        v[i + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = c0 + i < nlist ? row[c0 + i] : 0xffffffffu;
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = c0 + i < nlist ? row[c0 + i] : 0xffffffffu;
-  }
+  };
+  load_row(COAL);
   // the kept keys, sorted: runs of W keys by the waves, then merge levels;
   // the first np go out
   auto sort_and_emit = [&](uint32_t M) {
@@ -786,7 +808,7 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
     uint32_t mv = 0xffffffffu;
 #pragma unroll
     for (int i = 0; i < V; ++i)
-      if (c0 + i < nlist) mv = min(mv, v[i]);
+      if (lid(i) < nlist) mv = min(mv, v[i]);
     const uint32_t K = (np + 3) / 4;
     uint32_t T = 0;  // bits above b of the K-th smallest
     for (int b = 31; b >= 0; --b) {
@@ -800,7 +822,7 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
     uint32_t cnt = 0;
 #pragma unroll
     for (int i = 0; i < V; ++i)
-      if (c0 + i < nlist) cnt += ((static_cast<uint64_t>(v[i]) << 32) | (c0 + i)) <= tk ? 1u : 0u;
+      if (lid(i) < nlist) cnt += ((static_cast<uint64_t>(v[i]) << 32) | lid(i)) <= tk ? 1u : 0u;
     uint32_t off = block_exscan256(cnt, tmp);
     if (tid == 255) sh_rank = off + cnt;
     __syncthreads();
@@ -810,8 +832,8 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
       while (C < total) C <<= 1;
 #pragma unroll
       for (int i = 0; i < V; ++i)
-        if (c0 + i < nlist) {
-          const uint64_t key = (static_cast<uint64_t>(v[i]) << 32) | (c0 + i);
+        if (lid(i) < nlist) {
+          const uint64_t key = (static_cast<uint64_t>(v[i]) << 32) | lid(i);
           if (key <= tk) keys[off++] = key;
         }
       for (uint32_t p = total + tid; p < C; p += 256) keys[p] = kNoKey;
@@ -820,6 +842,7 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
       return;
     }
     __syncthreads();
+    if (COAL) load_row(false);
   }
   // range of the row (lists past nlist hold 0xffffffff, above every distance)
   uint32_t vmin = 0xffffffffu, vmax = 0;
@@ -1424,12 +1447,17 @@ static void select_reg_launch(const float *dist, uint32_t B, uint32_t nlist, uin
   while (M < np) M <<= 1;
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_select_reg<V>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_select_reg<V, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(2 * SEL_MAXNP * 8));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_select_reg<V, true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(2 * SEL_MAXNP * 8));
     attr = true;
   }
   const uint32_t words = np <= 256 && M < SEL_FAST ? 2 * SEL_FAST : 2 * M;  // the sort's two buffers
-  hipLaunchKernelGGL(k_select_reg<V>, dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
+  if (np <= 256 && nlist == 256u * V)
+    hipLaunchKernelGGL((k_select_reg<V, true>), dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
+  else
+    hipLaunchKernelGGL((k_select_reg<V, false>), dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
 }
 
 void launch_select(const float *dist, uint32_t B, uint32_t nlist, uint32_t np, uint32_t *probes, float *probe_d,
