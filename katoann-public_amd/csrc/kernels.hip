@@ -743,7 +743,7 @@ template <int V, bool COAL>
 __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__ dist, uint32_t B, uint32_t nlist,
                                                     uint32_t np, uint32_t *__restrict__ probes,
                                                     float *__restrict__ probe_d) {
-  extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x max(M, SEL_FAST if np <= 256)
+  extern __shared__ __attribute__((aligned(16))) uint64_t keys[];  // 2 x max(M, SEL_FAST if np <= 512)
   __shared__ uint32_t hist[SEL_BINS];  // one for the four waves: LDS for four blocks per CU
   __shared__ uint32_t tmp[8], sh_digit, sh_rank;
   const uint32_t q = blockIdx.x;
@@ -753,12 +753,12 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
   const uint32_t *row = reinterpret_cast<const uint32_t *>(dist) + static_cast<size_t>(q) * nlist;
   const uint32_t c0 = static_cast<uint32_t>(tid) * V;
   uint32_t v[V];
-  // Small np on a full row: the wave reads 1 KB runs (lane-contiguous 16-byte
+  // np <= 512 on a full row: the wave reads 1 KB runs (lane-contiguous 16-byte
   // pieces; thread t's value i is list 4 (256 (i / 4) + t) + i % 4): a few
   // cache lines per load instead of one per lane. The small-np path only
   // compares (value, list) keys, so any ownership serves; the radix path
   // below reloads the row thread-contiguous.
-  // (COAL: np <= 256 and nlist = 256 V, chosen at launch)
+  // (COAL: np <= 512 and nlist = 256 V, chosen at launch)
   auto lid = [&](int i) -> uint32_t {
     return COAL ? 4u * (256u * static_cast<uint32_t>(i >> 2) + static_cast<uint32_t>(tid)) + static_cast<uint32_t>(i & 3)
                 : c0 + static_cast<uint32_t>(i);
@@ -798,22 +798,37 @@ __global__ __launch_bounds__(256, 2) void k_select_reg(const float *__restrict__
       probe_d[static_cast<size_t>(q) * np + p] = __uint_as_float(static_cast<uint32_t>(k >> 32));
     }
   };
-  if (np <= 256) {
-    // Each wave finds the ceil(np / 4)-th smallest of its threads' minimum
-    // distances (one per lane; bit by bit from the top, one ballot per bit);
-    // at least np keys lie at or below the largest of the four, and for rows
-    // without long runs of small values in few threads not many more: those
-    // candidates hold the answer. Past SEL_FAST candidates the radix path
-    // below runs instead.
-    uint32_t mv = 0xffffffffu;
+  if (np <= 512) {
+    // Each wave finds the ceil(np / 4)-th smallest of its threads' R
+    // smallest distances (R = 1, 2 or 4, so that they number at least twice
+    // that; bit by bit from the top, one ballot per bit and value); they are
+    // distinct lists, so at least np keys lie at or below the largest of the
+    // four bounds, and for rows without long runs of small values in few
+    // threads not many more: those candidates hold the answer. Past SEL_FAST
+    // candidates the radix path below runs instead.
+    const uint32_t K = (np + 3) / 4, R = K <= 32 ? 1u : K <= 64 ? 2u : 4u;
+    uint32_t m0 = 0xffffffffu, m1 = 0xffffffffu, m2 = 0xffffffffu, m3 = 0xffffffffu;  // ascending
+    if (R == 1) {
 #pragma unroll
-    for (int i = 0; i < V; ++i)
-      if (lid(i) < nlist) mv = min(mv, v[i]);
-    const uint32_t K = (np + 3) / 4;
+      for (int i = 0; i < V; ++i)
+        if (lid(i) < nlist) m0 = min(m0, v[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (lid(i) < nlist) {
+          m3 = min(m3, max(m2, v[i]));
+          m2 = min(m2, max(m1, v[i]));
+          m1 = min(m1, max(m0, v[i]));
+          m0 = min(m0, v[i]);
+        }
+    }
     uint32_t T = 0;  // bits above b of the K-th smallest
     for (int b = 31; b >= 0; --b) {
       const uint32_t hi0 = T | ((1u << b) - 1u);  // largest value with bit b clear under the prefix
-      if (static_cast<uint32_t>(__popcll(ballot(mv <= hi0))) < K) T |= 1u << b;
+      uint32_t c = static_cast<uint32_t>(__popcll(ballot(m0 <= hi0)));
+      if (R > 1) c += static_cast<uint32_t>(__popcll(ballot(m1 <= hi0)));
+      if (R > 2) c += static_cast<uint32_t>(__popcll(ballot(m2 <= hi0))) + static_cast<uint32_t>(__popcll(ballot(m3 <= hi0)));
+      if (c < K) T |= 1u << b;
     }
     const uint64_t wk = (static_cast<uint64_t>(T) << 32) | 0xffffffffull;
     if (lane == 0) keys[w] = wk;
@@ -1453,8 +1468,8 @@ static void select_reg_launch(const float *dist, uint32_t B, uint32_t nlist, uin
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(2 * SEL_MAXNP * 8));
     attr = true;
   }
-  const uint32_t words = np <= 256 && M < SEL_FAST ? 2 * SEL_FAST : 2 * M;  // the sort's two buffers
-  if (np <= 256 && nlist == 256u * V)
+  const uint32_t words = np <= 512 && M < SEL_FAST ? 2 * SEL_FAST : 2 * M;  // the sort's two buffers
+  if (np <= 512 && nlist == 256u * V)
     hipLaunchKernelGGL((k_select_reg<V, true>), dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
   else
     hipLaunchKernelGGL((k_select_reg<V, false>), dim3(B), dim3(256), words * 8, s, dist, B, nlist, np, probes, probe_d);
