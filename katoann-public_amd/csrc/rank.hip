@@ -1714,7 +1714,9 @@ __device__ __forceinline__ bool update_by_selection(uint32_t S, uint32_t n) {
 
 // Selection path (pools over max(2 S, 256) keys): at least S keys lie at or
 // below the largest of the waves' ceil(S / 4)-th smallest thread minimum (one
-// per lane), and for pools in no particular order not many more;
+// per lane; each lane's two smallest when ceil(S / 4) > 32, so that the wave
+// picks among twice as many), and for pools in no particular order not many
+// more;
 // those candidates, sorted by the four waves, give the S smallest. A query
 // with more than UPD_CAP candidates is left to k_update (run after it).
 // Each thread's keys are loaded at once (independent loads) into registers;
@@ -1734,15 +1736,25 @@ __global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
     const uint32_t i = tid + 256u * r;
     v[r] = i < n ? pq[i] : kNoKey;
   }
-  // the wave's ceil(S / 4)-th smallest thread minimum (distance bits), found
-  // bit by bit from the top with one ballot per bit
-  uint32_t mv = 0xffffffffu;
-#pragma unroll
-  for (int r = 0; r < UPD_R; ++r) mv = min(mv, static_cast<uint32_t>(v[r] >> 32));
+  // the wave's ceil(S / 4)-th smallest thread minimum (distance bits; among
+  // the lanes' two smallest for K > 32), found bit by bit from the top with
+  // one ballot per bit and value
   const uint32_t K = (a.S + 3) / 4;
+  const bool two = K > 32;
+  uint32_t mv = 0xffffffffu, mv2 = 0xffffffffu;
+#pragma unroll
+  for (int r = 0; r < UPD_R; ++r) {
+    const uint32_t dv = static_cast<uint32_t>(v[r] >> 32);
+    if (two) mv2 = min(mv2, max(mv, dv));
+    mv = min(mv, dv);
+  }
   uint32_t T = 0;
-  for (int b = 31; b >= 0; --b)
-    if (static_cast<uint32_t>(__popcll(ballot(mv <= (T | ((1u << b) - 1u))))) < K) T |= 1u << b;
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t hi0 = T | ((1u << b) - 1u);
+    uint32_t c = static_cast<uint32_t>(__popcll(ballot(mv <= hi0)));
+    if (two) c += static_cast<uint32_t>(__popcll(ballot(mv2 <= hi0)));
+    if (c < K) T |= 1u << b;
+  }
   const uint64_t wk = (static_cast<uint64_t>(T) << 32) | 0xffffffffull;
   if (lane == 0) swk[w] = wk;
   __syncthreads();
