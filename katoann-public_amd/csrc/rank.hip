@@ -1721,6 +1721,7 @@ __device__ __forceinline__ bool update_by_selection(uint32_t S, uint32_t n) {
 // with more than UPD_CAP candidates is left to k_update (run after it).
 // Each thread's keys are loaded at once (independent loads) into registers;
 // real keys are below kNoKey.
+template <bool TWO>  // TWO: S > 128 (each lane's two smallest)
 __global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
   __shared__ uint64_t skeys[2 * UPD_CAP];
   __shared__ uint64_t swk[4];
@@ -1740,19 +1741,18 @@ __global__ __launch_bounds__(256) void k_update_sel(UpdateArgs a) {
   // the lanes' two smallest for K > 32), found bit by bit from the top with
   // one ballot per bit and value
   const uint32_t K = (a.S + 3) / 4;
-  const bool two = K > 32;
   uint32_t mv = 0xffffffffu, mv2 = 0xffffffffu;
 #pragma unroll
   for (int r = 0; r < UPD_R; ++r) {
     const uint32_t dv = static_cast<uint32_t>(v[r] >> 32);
-    if (two) mv2 = min(mv2, max(mv, dv));
+    if (TWO) mv2 = min(mv2, max(mv, dv));
     mv = min(mv, dv);
   }
   uint32_t T = 0;
   for (int b = 31; b >= 0; --b) {
     const uint32_t hi0 = T | ((1u << b) - 1u);
     uint32_t c = static_cast<uint32_t>(__popcll(ballot(mv <= hi0)));
-    if (two) c += static_cast<uint32_t>(__popcll(ballot(mv2 <= hi0)));
+    if (TWO) c += static_cast<uint32_t>(__popcll(ballot(mv2 <= hi0)));
     if (c < K) T |= 1u << b;
   }
   const uint64_t wk = (static_cast<uint64_t>(T) << 32) | 0xffffffffull;
@@ -2729,7 +2729,8 @@ bool launch_update(uint64_t *pool, uint32_t *pool_cnt, uint32_t *pool_sorted, ui
     return true;
   }
   // large pools by selection, then the rest (and any the selection left) by merging
-  if (S <= 256) hipLaunchKernelGGL(k_update_sel, grid, dim3(256), 0, s, a);
+  if (S <= 128) hipLaunchKernelGGL(k_update_sel<false>, grid, dim3(256), 0, s, a);
+  else if (S <= 256) hipLaunchKernelGGL(k_update_sel<true>, grid, dim3(256), 0, s, a);
   switch (refine_cap(S)) {
     case 64: hipLaunchKernelGGL(k_update<64>, grid, dim3(256), 0, s, a); return true;
     case 128: hipLaunchKernelGGL(k_update<128>, grid, dim3(256), 0, s, a); return true;
